@@ -1,0 +1,216 @@
+"""cpfft_amd -- MI355X implementation of CPFFT's finite-strain FFT solver step.
+
+Host-side Python mirror of the reference's operator interface over the
+C-ABI of libcpfft_amd.so (include/cpfft.h) and the Fortran load-step driver
+libcpfft_driver.so (cpfft_compute).  There is no CPU path: loading fails
+loudly when the native libraries are missing, and the operators need a HIP
+device.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .configs import WORKLOADS, Material, Workload  # noqa: F401
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(PKG, "libcpfft_amd.so")
+DRIVER = os.path.join(PKG, "libcpfft_driver.so")
+
+FN, FN1, PN, PN1, DFM, B, K4, STATE_N, STATE_NP1, WORK = range(10)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class CMaterial(ctypes.Structure):
+    _fields_ = [
+        ("slip_set", ctypes.c_int32),
+        ("e", ctypes.c_double), ("nu", ctypes.c_double), ("mu", ctypes.c_double),
+        ("rate_n", ctypes.c_double), ("tau_y", ctypes.c_double), ("tau_v", ctypes.c_double),
+        ("theta_0", ctypes.c_double), ("voche_m", ctypes.c_double),
+        ("gamma_bar", ctypes.c_double), ("alter_mode", ctypes.c_int32),
+        ("atol", ctypes.c_double), ("atol1", ctypes.c_double), ("rtol", ctypes.c_double),
+        ("rtol1", ctypes.c_double), ("miter", ctypes.c_int32), ("jac_exact", ctypes.c_int32),
+    ]
+
+
+# every symbol include/cpfft.h declares
+EXPORTS = [
+    "cpfft_create", "cpfft_destroy", "cpfft_last_error", "cpfft_nslip", "cpfft_field",
+    "cpfft_upload", "cpfft_download", "cpfft_reset", "cpfft_sync", "cpfft_gkdf", "cpfft_pcg",
+    "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
+    "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_counters",
+    "cpfft_kernel_times", "cpfft_set_timing",
+]
+
+_lib = None
+_drv = None
+
+
+def lib():
+    """Load libcpfft_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is missing: build the HIP library first "
+                               "(python -m cpfft_amd.build)")
+        L = ctypes.CDLL(LIB)
+        L.cpfft_create.argtypes = [ctypes.c_int, ctypes.POINTER(CMaterial), _dp, ctypes.c_double,
+                                   ctypes.c_int, ctypes.POINTER(_vp)]
+        L.cpfft_destroy.argtypes = [_vp]
+        L.cpfft_last_error.restype = ctypes.c_char_p
+        L.cpfft_nslip.argtypes = [_vp]
+        L.cpfft_field.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_field.restype = _vp
+        L.cpfft_upload.argtypes = [_vp, ctypes.c_int, _dp]
+        L.cpfft_download.argtypes = [_vp, ctypes.c_int, _dp]
+        L.cpfft_reset.argtypes = [_vp]
+        L.cpfft_sync.argtypes = [_vp]
+        L.cpfft_gkdf.argtypes = [_vp, _vp, _vp, ctypes.c_int]
+        L.cpfft_pcg.argtypes = [_vp, _vp, _vp, ctypes.c_double, _ip, _dp]
+        L.cpfft_drive_eps_sig.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.cpfft_tangent_homo.argtypes = [_vp, ctypes.c_double, _dp]
+        L.cpfft_update.argtypes = [_vp]
+        L.cpfft_dnrm2.argtypes = [_vp, _vp, _dp]
+        L.cpfft_daxpy.argtypes = [_vp, ctypes.c_double, _vp, _vp]
+        L.cpfft_dscal.argtypes = [_vp, ctypes.c_double, _vp]
+        L.cpfft_dcopy.argtypes = [_vp, _vp, _vp]
+        L.cpfft_fill9.argtypes = [_vp, _vp, _dp]
+        L.cpfft_mean9.argtypes = [_vp, _vp, _dp]
+        L.cpfft_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong)]
+        L.cpfft_kernel_times.argtypes = [_vp, _dp]
+        L.cpfft_set_timing.argtypes = [_vp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def driver():
+    """Load the Fortran driver libcpfft_driver.so (raises if not built)."""
+    global _drv
+    if _drv is None:
+        lib()
+        if not os.path.exists(DRIVER):
+            raise RuntimeError(f"{DRIVER} is missing: build it first (python -m cpfft_amd.build)")
+        D = ctypes.CDLL(DRIVER)
+        D.cpfft_compute.argtypes = [_vp, ctypes.c_int, _dp, _ip, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _ip, _dp]
+        _drv = D
+    return _drv
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class CpfftError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise CpfftError(f"{what} failed ({rc}): {lib().cpfft_last_error().decode()}")
+    return rc
+
+
+class Solver:
+    """One RVE on one GPU: the reference's module `fft` state in HBM plus its
+    operators (G_K_dF, fftPcg, drive_eps_sig, tangent_homo, update)."""
+
+    def __init__(self, N, angles, material=None, dt=0.5, gamma_mode=1):
+        m = material or Material()
+        cm = CMaterial(**{k: getattr(m, k) for k, _ in CMaterial._fields_})
+        self.N = N
+        self.V = N ** 3
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        assert ang.shape == (3, self.V), "angles must be [3][N^3]"
+        h = _vp()
+        _check(lib().cpfft_create(N, ctypes.byref(cm), _p(ang), dt, gamma_mode, ctypes.byref(h)),
+               "cpfft_create")
+        self.h = h
+        self.nslip = lib().cpfft_nslip(h)
+        self.nwords = 26 + 2 * self.nslip
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cpfft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def field(self, f):
+        return lib().cpfft_field(self.h, f)
+
+    def _len(self, f):
+        return {K4: 81, STATE_N: self.nwords, STATE_NP1: self.nwords}.get(f, 9) * self.V
+
+    def upload(self, f, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self._len(f)
+        _check(lib().cpfft_upload(self.h, f, _p(a)), "cpfft_upload")
+
+    def download(self, f):
+        out = np.empty(self._len(f))
+        _check(lib().cpfft_download(self.h, f, _p(out)), "cpfft_download")
+        rows = self._len(f) // self.V
+        return out.reshape(rows, self.V)
+
+    def reset(self):
+        _check(lib().cpfft_reset(self.h), "cpfft_reset")
+
+    def gkdf(self, src, dst, flgK):
+        _check(lib().cpfft_gkdf(self.h, self.field(src), self.field(dst), int(flgK)), "cpfft_gkdf")
+
+    def pcg(self, b, x, tol):
+        it = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        rc = lib().cpfft_pcg(self.h, self.field(b), self.field(x), tol, ctypes.byref(it),
+                             ctypes.byref(rel))
+        if rc < -1:
+            raise CpfftError(lib().cpfft_last_error().decode())
+        return rc, it.value, rel.value
+
+    def drive_eps_sig(self, step, iiter):
+        return _check(lib().cpfft_drive_eps_sig(self.h, step, iiter), "cpfft_drive_eps_sig")
+
+    def tangent_homo(self, tol):
+        C = np.zeros(81)
+        _check(lib().cpfft_tangent_homo(self.h, tol, _p(C)), "cpfft_tangent_homo")
+        return C
+
+    def update(self):
+        _check(lib().cpfft_update(self.h), "cpfft_update")
+
+    def sync(self):
+        _check(lib().cpfft_sync(self.h), "cpfft_sync")
+
+    def counters(self):
+        out = (ctypes.c_longlong * 8)()
+        _check(lib().cpfft_counters(self.h, out), "cpfft_counters")
+        return list(out)
+
+    def set_timing(self, on):
+        lib().cpfft_set_timing(self.h, int(on))
+
+    def kernel_times(self):
+        out = np.zeros(4)
+        lib().cpfft_kernel_times(self.h, _p(out))
+        return out
+
+    def compute(self, BC_all, isNBC, tolPCG=1e-8, tolNR=1e-6, maxIter=25, verbose=False):
+        """The `compute` command: drive_eps_sig(1,0) then the FFT_nr3 load-step
+        loop, run by the Fortran driver.  Returns (rc, Pbar[nstep,9],
+        counters[nstep,4], times[3])."""
+        BC = np.ascontiguousarray(BC_all, dtype=np.float64)
+        nstep = BC.shape[0]
+        nbc = np.ascontiguousarray(isNBC, dtype=np.int32)
+        Pbar = np.zeros((nstep, 9))
+        cnt = np.zeros((nstep, 4), dtype=np.int32)
+        times = np.zeros(3)
+        rc = driver().cpfft_compute(self.h, nstep, _p(BC), nbc.ctypes.data_as(_ip), tolPCG, tolNR,
+                                    maxIter, int(verbose), _p(Pbar), cnt.ctypes.data_as(_ip),
+                                    _p(times))
+        return rc, Pbar, cnt, times

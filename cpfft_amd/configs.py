@@ -1,0 +1,86 @@
+"""Workload definitions: material, microstructure and loading of the FCC
+crystal-plasticity benchmarks, in the semantics of the reference's deck
+(crystal / material / strains / loading / nonlinear analysis parameters).
+
+Loading follows inlod (inlod.f:57-63): BC_all(:,step) is the cumulative sum
+of FP_max * mults(step) plus the identity on prescribed-F diagonal terms.
+Uniaxial tension along x: F_xx prescribed, P_yy = P_zz = 0 prescribed,
+every other component of F prescribed with zero increment.
+
+Time stepping: the FCC material (rate exponent 20, Voce tau_y = 16 MPa,
+theta_0 = 250 MPa, reference slip rate 1e-3/s) is loaded at 1e-3/s with a
+0.5 s step (5e-4 strain per step).  With a 1 s step the reference's global
+Newton iteration (no line search) does not converge for this material.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .microstructure import polycrystal
+
+# FCC cubic elasticity C11/C12/C44 = 168.4/121.4/75.4 GPa expressed through the
+# (e, nu, mu) inputs of elastic_type cubic (mod_crystals.f:1833-1869)
+C11, C12, C44 = 168400.0, 121400.0, 75400.0
+E_CUBIC = (C11 - C12) * (C11 + 2 * C12) / (C11 + C12)
+NU_CUBIC = C12 / (C11 + C12)
+
+
+@dataclass
+class Material:
+    slip_set: int = 1           # 1 fcc, 8 bcc {110}+{112}
+    e: float = E_CUBIC
+    nu: float = NU_CUBIC
+    mu: float = C44
+    rate_n: float = 20.0
+    tau_y: float = 16.0
+    tau_v: float = 132.0
+    theta_0: float = 250.0
+    voche_m: float = 1.0
+    gamma_bar: float = 1.0e-3
+    alter_mode: int = 1
+    atol: float = 1.0e-5
+    atol1: float = 1.0e-5
+    rtol: float = 5.0e-5
+    rtol1: float = 1.0e-5
+    miter: int = 30
+    jac_exact: int = 1
+
+
+@dataclass
+class Workload:
+    name: str
+    N: int
+    ngrains: int
+    seed: int
+    nstep: int
+    strain: float               # final F_xx - 1
+    dt: float = 0.5
+    tolPCG: float = 1.0e-8
+    tolNR: float = 1.0e-6
+    maxIter: int = 25
+    gamma_mode: int = 1         # 1 textbook projection, 0 reference operator
+    material: Material = field(default_factory=Material)
+
+    def angles(self):
+        return polycrystal(self.N, self.ngrains, self.seed)[1]
+
+    def bc(self):
+        """(BC_all [nstep][9], isNBC [9]) for uniaxial tension along x"""
+        isNBC = np.zeros(9, np.int32)
+        isNBC[[4, 8]] = 1
+        FP_max = np.zeros(9)
+        FP_max[0] = self.strain
+        mults = np.full(self.nstep, 1.0 / self.nstep)
+        BC = np.cumsum(FP_max[None, :] * mults[:, None], axis=0)
+        for d in (0, 4, 8):
+            if not isNBC[d]:
+                BC[:, d] += 1.0
+        return np.ascontiguousarray(BC), isNBC
+
+
+# seeds: numpy default_rng(20261202 + k) for workload k
+WORKLOADS = {
+    "c1_16": Workload("c1_16", 16, 8, 20261202 + 1, 40, 0.02),
+    "c2_64": Workload("c2_64", 64, 64, 20261202 + 2, 80, 0.04),
+    "c3_128": Workload("c3_128", 128, 256, 20261202 + 3, 120, 0.06),
+}

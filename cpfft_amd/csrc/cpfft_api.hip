@@ -1,0 +1,589 @@
+// cpfft_api.hip -- context, host-side setup and the C-ABI entry points of
+// libcpfft_amd.so (declared in include/cpfft.h).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cpfft.h"
+#include "kernels.h"
+
+using namespace cpfft;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* what) {
+  g_err = what;
+  return code;
+}
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+      return -100;                                                                \
+    }                                                                             \
+  } while (0)
+
+struct cpfft_ctx {
+  int N = 0, Nz = 0;
+  long V = 0;
+  int nslip = 0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  SpecGeom geom{};
+  MatParams mp{};
+  double* fields[CPFFT_NFIELDS] = {};
+  long field_len[CPFFT_NFIELDS] = {};
+  cplx* spec = nullptr;
+  cplx* tw = nullptr;
+  double* r = nullptr;   // CG residual
+  double* p = nullptr;   // CG direction
+  double* q = nullptr;   // CG operator image
+  double* tmp9 = nullptr;  // tangent_homo right-hand side
+  double* partials = nullptr;
+  double* dres = nullptr;  // device results of reductions (16 doubles)
+  double* hres = nullptr;  // pinned host mirror
+  void* orients = nullptr;
+  int32_t* oid = nullptr;
+  long long* dcnt = nullptr;
+  long long cg_iters = 0, gkdf_calls = 0, des_calls = 0;
+  int timing = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double t_ms[4] = {0, 0, 0, 0};
+  long t_n[4] = {0, 0, 0, 0};
+  double t_last[4] = {0, 0, 0, 0};
+};
+
+// ------------------------------------------------------------------ setup
+// crystal-frame slip systems: mod_crystals.f:438-515 (fcc) and the first 24
+// systems of bcc48, mod_crystals.f:812-1207 ({110} then {112})
+static void slip_systems(int set, int& ns, double b[24][3], double n[24][3]) {
+  if (set == 1) {
+    static const int bb[12][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}, {0, 1, 1},  {1, 0, 1},  {1, -1, 0},
+                                  {0, -1, 1}, {-1, 0, -1}, {1, 1, 0}, {0, 1, 1},  {1, 0, -1}, {-1, -1, 0}};
+    static const int nn[12][3] = {{1, 1, 1},   {1, 1, 1},   {1, 1, 1},   {-1, -1, 1}, {-1, -1, 1}, {-1, -1, 1},
+                                  {-1, 1, 1},  {-1, 1, 1},  {-1, 1, 1},  {1, -1, 1},  {1, -1, 1},  {1, -1, 1}};
+    ns = 12;
+    const double f2 = 1 / sqrt(2.0), f3 = 1 / sqrt(3.0);
+    for (int s = 0; s < 12; ++s)
+      for (int k = 0; k < 3; ++k) { b[s][k] = bb[s][k] * f2; n[s][k] = nn[s][k] * f3; }
+  } else {
+    static const int bb[24][3] = {{-1, 1, 1}, {1, -1, 1}, {1, 1, 1},  {1, 1, -1}, {1, 1, -1}, {-1, 1, 1},
+                                  {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, -1, 1}, {1, 1, 1},  {-1, 1, 1},
+                                  {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1},
+                                  {1, 1, 1},  {-1, 1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1}};
+    static const int nn[24][3] = {{1, 1, 0},  {1, 1, 0},  {1, -1, 0}, {1, -1, 0}, {1, 0, 1},  {1, 0, 1},
+                                  {1, 0, -1}, {1, 0, -1}, {0, 1, 1},  {0, 1, 1},  {0, 1, -1}, {0, 1, -1},
+                                  {1, 1, 2},  {-1, 1, 2}, {1, -1, 2}, {1, 1, -2}, {1, 2, 1},  {-1, 2, 1},
+                                  {1, -2, 1}, {1, 2, -1}, {2, 1, 1},  {-2, 1, 1}, {2, -1, 1}, {2, 1, -1}};
+    ns = 24;
+    const double f2 = 1.0 / sqrt(2.0), f3 = 1.0 / sqrt(3.0), f112 = 1.0 / sqrt(6.0), f211 = 2.0 / sqrt(6.0);
+    for (int s = 0; s < 24; ++s)
+      for (int k = 0; k < 3; ++k) {
+        b[s][k] = bb[s][k] * f3;
+        int v = nn[s][k];
+        n[s][k] = s < 12 ? v * f2 : (v == 2 ? f211 : v == -2 ? -f211 : v * f112);
+      }
+  }
+}
+
+// 6x6 inverse by Gauss-Jordan with partial pivoting (compliance -> stiffness)
+static bool inv6(double A[6][6]) {
+  double M[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 12; ++j) M[i][j] = j < 6 ? A[i][j] : (j - 6 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 6; ++c) {
+    int p = c;
+    for (int i = c + 1; i < 6; ++i)
+      if (fabs(M[i][c]) > fabs(M[p][c])) p = i;
+    if (M[p][c] == 0.0) return false;
+    for (int j = 0; j < 12; ++j) std::swap(M[c][j], M[p][j]);
+    double d = M[c][c];
+    for (int j = 0; j < 12; ++j) M[c][j] /= d;
+    for (int i = 0; i < 6; ++i)
+      if (i != c) {
+        double f = M[i][c];
+        for (int j = 0; j < 12; ++j) M[i][j] -= f * M[c][j];
+      }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) A[i][j] = M[i][j + 6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < i; ++j) A[i][j] = A[j][i];
+  return true;
+}
+
+// Kocks angles (degrees) -> rotation g (mm10_rotation_matrix, mm10_a.f:1287-1345)
+static void kocks_g(const double a[3], double g[3][3]) {
+  const double pi = 3.141592653589793;
+  double psi = a[0] * pi / 180.0, th = a[1] * pi / 180.0, phi = a[2] * pi / 180.0;
+  g[0][0] = -sin(psi) * sin(phi) - cos(psi) * cos(phi) * cos(th);
+  g[0][1] = cos(psi) * sin(phi) - sin(psi) * cos(phi) * cos(th);
+  g[0][2] = cos(phi) * sin(th);
+  g[1][0] = sin(psi) * cos(phi) - cos(psi) * sin(phi) * cos(th);
+  g[1][1] = -cos(psi) * cos(phi) - sin(psi) * sin(phi) * cos(th);
+  g[1][2] = sin(phi) * sin(th);
+  g[2][0] = cos(psi) * sin(th);
+  g[2][1] = sin(psi) * sin(th);
+  g[2][2] = cos(th);
+}
+
+template <int NS>
+static void build_orient(const double stiff[6][6], const double b[24][3], const double n[24][3],
+                         const double ang[3], Orient<NS>& o) {
+  double g[3][3], t[3][3];
+  kocks_g(ang, g);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t[i][j] = g[j][i];
+  for (int s = 0; s < NS; ++s) {
+    double bs[3], nv[3];
+    for (int i = 0; i < 3; ++i) {
+      bs[i] = t[i][0] * b[s][0] + t[i][1] * b[s][1] + t[i][2] * b[s][2];
+      nv[i] = t[i][0] * n[s][0] + t[i][1] * n[s][1] + t[i][2] * n[s][2];
+    }
+    double A[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) A[i][j] = bs[i] * nv[j];
+    auto sy = [&](int i, int j) { return 0.5 * (A[i][j] + A[j][i]); };
+    auto sk = [&](int i, int j) { return 0.5 * (A[i][j] - A[j][i]); };
+    o.ms[s][0] = sy(0, 0);
+    o.ms[s][1] = sy(1, 1);
+    o.ms[s][2] = sy(2, 2);
+    o.ms[s][3] = 2.0 * sy(0, 1);
+    o.ms[s][4] = 2.0 * sy(1, 2);
+    o.ms[s][5] = 2.0 * sy(0, 2);
+    o.qs[s][0] = sk(1, 2);
+    o.qs[s][1] = sk(0, 2);
+    o.qs[s][2] = sk(0, 1);
+  }
+  double Rs[6][6], tmp[6][6];
+  rt2rve(t, Rs);
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double v = 0.0;
+      for (int k = 0; k < 6; ++k) v += stiff[i][k] * Rs[j][k];
+      tmp[i][j] = v;
+    }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double v = 0.0;
+      for (int k = 0; k < 6; ++k) v += Rs[i][k] * tmp[k][j];
+      o.C[i * 6 + j] = v;
+    }
+}
+
+template <int NS>
+static int upload_orients(cpfft_ctx* c, const std::vector<std::array<double, 3>>& uniq,
+                          const double stiff[6][6], const double b[24][3], const double n[24][3]) {
+  std::vector<Orient<NS>> h(uniq.size());
+  for (size_t k = 0; k < uniq.size(); ++k) build_orient<NS>(stiff, b, n, uniq[k].data(), h[k]);
+  HIP_OK(hipMalloc(&c->orients, h.size() * sizeof(Orient<NS>)));
+  HIP_OK(hipMemcpy(c->orients, h.data(), h.size() * sizeof(Orient<NS>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" const char* cpfft_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cpfft_destroy(cpfft_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (int f = 0; f < CPFFT_NFIELDS; ++f)
+    if (c->fields[f]) hipFree(c->fields[f]);
+  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->partials, c->dres, c->orients, c->oid, c->dcnt};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (c->hres) hipHostFree(c->hres);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int cpfft_reset(cpfft_ctx* c);
+
+extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles, double dt,
+                            int gamma_mode, cpfft_ctx** out) {
+  *out = nullptr;
+  if (N < 2 || N > 256) return fail(-1, "cpfft_create: N must be in [2, 256]");
+  if (!m || !angles) return fail(-1, "cpfft_create: null material or angles");
+  if (m->slip_set != 1 && m->slip_set != 8) return fail(-1, "cpfft_create: slip_set must be 1 (fcc) or 8 (bcc24)");
+  if (gamma_mode != 0 && gamma_mode != 1) return fail(-1, "cpfft_create: bad gamma_mode");
+  if (!(dt > 0.0)) return fail(-1, "cpfft_create: dt must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(-2, "cpfft_create: no HIP device available (this library has no CPU path)");
+  cpfft_ctx* c = new cpfft_ctx();
+  HIP_OK(hipGetDevice(&c->dev));
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->N = N;
+  c->Nz = N / 2 + 1;
+  c->V = (long)N * N * N;
+  const long V = c->V;
+  // geometry
+  int maxlds = 65536;
+  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->dev);
+  if (maxlds > 65536) maxlds = 65536;
+  SpecGeom& g = c->geom;
+  g.N = N;
+  g.Nz = c->Nz;
+  g.V = V;
+  g.S1 = (long)N * N * c->Nz;
+  g.plan = make_plan(N);
+  g.gamma_mode = gamma_mode;
+  g.ty = 8;
+  g.tz = 1;
+  while (g.tz < 4 && 2 * 9 * (long)(2 * g.tz) * N * (long)sizeof(cplx) <= maxlds) g.tz *= 2;
+  if (2 * 9 * (long)N * (long)sizeof(cplx) > maxlds) {
+    cpfft_destroy(c);
+    return fail(-3, "cpfft_create: grid edge too large for the LDS of this device");
+  }
+  for (int s = 0; s < g.plan.nst; ++s)
+    if (g.plan.rad[s] > 16) {
+      cpfft_destroy(c);
+      return fail(-3, "cpfft_create: N has a prime factor above 16");
+    }
+  // material
+  double b[24][3], nn[24][3];
+  slip_systems(m->slip_set, c->nslip, b, nn);
+  double st[6][6] = {};
+  st[0][0] = st[1][1] = st[2][2] = 1 / m->e;
+  st[0][1] = st[0][2] = st[1][0] = st[1][2] = st[2][0] = st[2][1] = -m->nu / m->e;
+  st[3][3] = st[4][4] = st[5][5] = 1 / m->mu;
+  if (!inv6(st)) {
+    cpfft_destroy(c);
+    return fail(-1, "cpfft_create: singular compliance");
+  }
+  MatParams& P = c->mp;
+  P.rate_n = m->rate_n;
+  P.tau_y = m->tau_y;
+  P.tau_v = m->tau_v;
+  P.theta_0 = m->theta_0;
+  P.voche_m = m->voche_m;
+  P.eps_dot_0_y = m->gamma_bar;
+  P.dt = dt;
+  P.atol = m->atol;
+  P.atol1 = m->atol1;
+  P.rtol = m->rtol;
+  P.rtol1 = m->rtol1;
+  P.miter = m->miter;
+  P.alter_mode = m->alter_mode;
+  P.jac_exact = m->jac_exact;
+  P.int_rate = (m->rate_n == floor(m->rate_n) && m->rate_n >= 2.0 && m->rate_n <= 64.0) ? 1 : 0;
+  P.m_is_one = (m->voche_m == 1.0) ? 1 : 0;
+  // orientations: deduplicate the per-voxel angles
+  std::map<std::tuple<double, double, double>, int32_t> ids;
+  std::vector<std::array<double, 3>> uniq;
+  std::vector<int32_t> hoid(V);
+  for (long e = 0; e < V; ++e) {
+    auto key = std::make_tuple(angles[e], angles[V + e], angles[2 * V + e]);
+    auto it = ids.find(key);
+    if (it == ids.end()) {
+      int32_t id = (int32_t)uniq.size();
+      ids.emplace(key, id);
+      uniq.push_back({angles[e], angles[V + e], angles[2 * V + e]});
+      hoid[e] = id;
+    } else {
+      hoid[e] = it->second;
+    }
+  }
+  int rc = c->nslip == 12 ? upload_orients<12>(c, uniq, st, b, nn) : upload_orients<24>(c, uniq, st, b, nn);
+  if (rc) { cpfft_destroy(c); return rc; }
+  HIP_OK(hipMalloc(&c->oid, V * sizeof(int32_t)));
+  HIP_OK(hipMemcpy(c->oid, hoid.data(), V * sizeof(int32_t), hipMemcpyHostToDevice));
+  // fields
+  const int W = CPFFT_NWORDS(c->nslip);
+  long lens[CPFFT_NFIELDS] = {9 * V, 9 * V, 9 * V, 9 * V, 9 * V, 9 * V, 81 * V, W * V, W * V, 9 * V};
+  for (int f = 0; f < CPFFT_NFIELDS; ++f) {
+    c->field_len[f] = lens[f];
+    HIP_OK(hipMalloc(&c->fields[f], lens[f] * sizeof(double)));
+  }
+  HIP_OK(hipMalloc(&c->spec, 9 * g.S1 * sizeof(cplx)));
+  HIP_OK(hipMalloc(&c->r, 9 * V * sizeof(double)));
+  HIP_OK(hipMalloc(&c->p, 9 * V * sizeof(double)));
+  HIP_OK(hipMalloc(&c->q, 9 * V * sizeof(double)));
+  HIP_OK(hipMalloc(&c->tmp9, 9 * V * sizeof(double)));
+  HIP_OK(hipMalloc(&c->partials, reduce_partials_count(9 * V) * sizeof(double)));
+  HIP_OK(hipMalloc(&c->dres, 16 * sizeof(double)));
+  HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipMalloc(&c->dcnt, 8 * sizeof(long long)));
+  HIP_OK(hipMemset(c->dcnt, 0, 8 * sizeof(long long)));
+  // twiddles exp(-2 pi i k/N), computed in long double
+  std::vector<cplx> htw(N);
+  const long double pil = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < N; ++k) {
+    long double a = 2.0L * pil * (long double)k / (long double)N;
+    htw[k] = cplx{(double)cosl(a), (double)-sinl(a)};
+  }
+  HIP_OK(hipMalloc(&c->tw, N * sizeof(cplx)));
+  HIP_OK(hipMemcpy(c->tw, htw.data(), N * sizeof(cplx), hipMemcpyHostToDevice));
+  HIP_OK(hipEventCreate(&c->ev0));
+  HIP_OK(hipEventCreate(&c->ev1));
+  rc = cpfft_reset(c);
+  if (rc) { cpfft_destroy(c); return rc; }
+  *out = c;
+  return 0;
+}
+
+extern "C" int cpfft_nslip(const cpfft_ctx* c) { return c ? c->nslip : -1; }
+
+extern "C" double* cpfft_field(cpfft_ctx* c, int f) {
+  if (!c || f < 0 || f >= CPFFT_NFIELDS) return nullptr;
+  return c->fields[f];
+}
+
+extern "C" int cpfft_upload(cpfft_ctx* c, int f, const double* h) {
+  if (!c || f < 0 || f >= CPFFT_NFIELDS) return fail(-1, "cpfft_upload: bad field");
+  HIP_OK(hipMemcpyAsync(c->fields[f], h, c->field_len[f] * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int cpfft_download(cpfft_ctx* c, int f, double* h) {
+  if (!c || f < 0 || f >= CPFFT_NFIELDS) return fail(-1, "cpfft_download: bad field");
+  HIP_OK(hipMemcpyAsync(h, c->fields[f], c->field_len[f] * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int cpfft_sync(cpfft_ctx* c) {
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int cpfft_reset(cpfft_ctx* c) {
+  const long V = c->V;
+  const int ns = c->nslip, W = CPFFT_NWORDS(ns);
+  std::vector<double> h(9 * V, 0.0);
+  for (long e = 0; e < V; ++e) h[e] = h[4 * V + e] = h[8 * V + e] = 1.0;
+  HIP_OK(hipMemcpy(c->fields[CPFFT_FN], h.data(), 9 * V * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->fields[CPFFT_FN1], h.data(), 9 * V * sizeof(double), hipMemcpyHostToDevice));
+  std::fill(h.begin(), h.end(), 0.0);
+  for (int f : {CPFFT_PN, CPFFT_PN1, CPFFT_DFM, CPFFT_B, CPFFT_WORK})
+    HIP_OK(hipMemcpy(c->fields[f], h.data(), 9 * V * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> s((long)W * V, 0.0);
+  for (long e = 0; e < V; ++e) {
+    s[(CPFFT_W_RP + 0) * V + e] = 1.0;
+    s[(CPFFT_W_RP + 4) * V + e] = 1.0;
+    s[(CPFFT_W_RP + 8) * V + e] = 1.0;
+    s[CPFFT_W_TT * V + e] = c->mp.tau_y + 1.0e-5;  // mm10_init_voche (mm10_a.f:2042-2055)
+  }
+  HIP_OK(hipMemcpy(c->fields[CPFFT_STATE_N], s.data(), W * V * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->fields[CPFFT_STATE_NP1], s.data(), W * V * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> z(81 * V, 0.0);
+  HIP_OK(hipMemcpy(c->fields[CPFFT_K4], z.data(), 81 * V * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ------------------------------------------------------------ timing
+static void tstart(cpfft_ctx* c) {
+  if (c->timing) hipEventRecord(c->ev0, c->stream);
+}
+static void tstop(cpfft_ctx* c, int fam) {
+  if (!c->timing) return;
+  hipEventRecord(c->ev1, c->stream);
+  hipEventSynchronize(c->ev1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->t_ms[fam] += ms;
+  c->t_n[fam] += 1;
+}
+
+extern "C" int cpfft_set_timing(cpfft_ctx* c, int on) {
+  c->timing = on;
+  for (int i = 0; i < 4; ++i) { c->t_ms[i] = 0; c->t_n[i] = 0; }
+  return 0;
+}
+
+extern "C" int cpfft_kernel_times(cpfft_ctx* c, double out[4]) {
+  for (int i = 0; i < 4; ++i) out[i] = c->t_n[i] ? c->t_ms[i] / c->t_n[i] : 0.0;
+  return 0;
+}
+
+// ------------------------------------------------------------ operators
+static int check_ptr(const void* p) { return p ? 0 : fail(-1, "null device pointer"); }
+
+extern "C" int cpfft_gkdf(cpfft_ctx* c, const double* F, double* GKF, int flgK) {
+  if (!c || check_ptr(F) || check_ptr(GKF)) return -1;
+  if (F == GKF) return fail(-1, "cpfft_gkdf: F and GKF must not alias");
+  tstart(c);
+  launch_gkdf(c->geom, F, flgK ? c->fields[CPFFT_K4] : nullptr, GKF, c->spec, c->tw, c->stream);
+  tstop(c, 0);
+  HIP_OK(hipGetLastError());
+  c->gkdf_calls++;
+  return 0;
+}
+
+static int read_res(cpfft_ctx* c, int n) {
+  HIP_OK(hipMemcpyAsync(c->hres, c->dres, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int dot_host(cpfft_ctx* c, const double* x, const double* y, double* out) {
+  launch_dot(x, y, 9 * c->V, c->partials, c->dres, c->stream);
+  int rc = read_res(c, 1);
+  if (rc) return rc;
+  *out = c->hres[0];
+  return 0;
+}
+
+extern "C" int cpfft_dnrm2(cpfft_ctx* c, const double* x, double* out) {
+  double d;
+  int rc = dot_host(c, x, x, &d);
+  if (rc) return rc;
+  *out = sqrt(d);
+  return 0;
+}
+
+extern "C" int cpfft_daxpy(cpfft_ctx* c, double a, const double* x, double* y) {
+  launch_axpy(9 * c->V, a, x, y, c->stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int cpfft_dscal(cpfft_ctx* c, double a, double* x) {
+  launch_scal(9 * c->V, a, x, c->stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int cpfft_dcopy(cpfft_ctx* c, const double* x, double* y) {
+  launch_copy(9 * c->V, x, y, c->stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int cpfft_fill9(cpfft_ctx* c, double* x, const double v[9]) {
+  launch_fill9(c->V, v, x, c->stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+extern "C" int cpfft_mean9(cpfft_ctx* c, const double* x, double out[9]) {
+  launch_mean9(x, c->V, c->partials, c->dres, c->stream);
+  int rc = read_res(c, 9);
+  if (rc) return rc;
+  for (int i = 0; i < 9; ++i) out[i] = c->hres[i];
+  return 0;
+}
+
+// fftPcg (FFT_nr3.f:214-360): textbook unpreconditioned CG, x0 = 0
+extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, int* iters,
+                         double* relres) {
+  const long n = 9 * c->V;
+  const int maxIter = 1000;
+  const double eps = 2.220446049250313e-16;
+  if (iters) *iters = 0;
+  if (relres) *relres = 0.0;
+  launch_scal(n, 0.0, x, c->stream);
+  double bb;
+  int rc = dot_host(c, b, b, &bb);
+  if (rc) return rc;
+  double n2b = sqrt(bb), tolb = tol * n2b;
+  if (tol <= eps || tol >= 1.0) return fail(-1, "cpfft_pcg: improper tolerance");
+  if (n2b <= eps) return 1;
+  launch_copy(n, b, c->r, c->stream);
+  launch_copy(n, b, c->p, c->stream);
+  double rr = bb, resnorm = n2b;
+  int k;
+  for (k = 1; k <= maxIter; ++k) {
+    tstart(c);
+    launch_gkdf(c->geom, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
+    tstop(c, 0);
+    c->gkdf_calls++;
+    double pq;
+    if ((rc = dot_host(c, c->p, c->q, &pq))) return rc;
+    double alpha = rr / pq;
+    tstart(c);
+    launch_cg_update(n, alpha, c->p, c->q, x, c->r, c->partials, c->dres, c->stream);
+    tstop(c, 2);
+    if ((rc = read_res(c, 1))) return rc;
+    double rr1 = c->hres[0];
+    resnorm = sqrt(rr1);
+    if (resnorm <= tolb || resnorm <= tol) break;
+    double beta = rr1 / rr;
+    rr = rr1;
+    launch_cg_dir(n, beta, c->r, c->p, c->stream);
+  }
+  if (k > maxIter) k = maxIter;
+  c->cg_iters += k;
+  if (iters) *iters = k;
+  if (relres) *relres = resnorm / n2b;
+  HIP_OK(hipGetLastError());
+  if (k >= maxIter) return fail(-2, "cpfft_pcg: no convergence within 1000 iterations");
+  return 0;
+}
+
+extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
+  long long before[8];
+  HIP_OK(hipMemcpy(before, c->dcnt, sizeof before, hipMemcpyDeviceToHost));
+  tstart(c);
+  if (c->nslip == 12)
+    launch_drive_eps_sig<12>(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step, iiter,
+                             c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                             c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->fields[CPFFT_K4],
+                             c->dcnt, c->stream);
+  else
+    launch_drive_eps_sig<24>(c->V, c->mp, (const Orient<24>*)c->orients, c->oid, step, iiter,
+                             c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                             c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->fields[CPFFT_K4],
+                             c->dcnt, c->stream);
+  tstop(c, 1);
+  HIP_OK(hipGetLastError());
+  long long after[8];
+  HIP_OK(hipMemcpyAsync(after, c->dcnt, sizeof after, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->des_calls++;
+  return (int)(after[3] - before[3]);
+}
+
+// tangent_homo (tangent_homo.f:11-73)
+extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
+  const long V = c->V;
+  double* C = c->fields[CPFFT_WORK];
+  double* b = c->tmp9;
+  for (int i = 0; i < 9; ++i) {
+    launch_gather_k4_col(V, i, c->fields[CPFFT_K4], C, c->stream);
+    tstart(c);
+    launch_gkdf(c->geom, C, nullptr, b, c->spec, c->tw, c->stream);
+    tstop(c, 0);
+    c->gkdf_calls++;
+    launch_scal(9 * V, -1.0, b, c->stream);
+    int it;
+    double rel;
+    int rc = cpfft_pcg(c, b, C, tol, &it, &rel);
+    if (rc < 0) return rc;
+    launch_add_identity_comp(V, i, C, c->stream);
+    launch_k4_dot9(V, c->fields[CPFFT_K4], C, c->partials, c->dres, c->stream);
+    if ((rc = read_res(c, 9))) return rc;
+    for (int j = 0; j < 9; ++j) C_homo[9 * j + i] = c->hres[j];
+  }
+  return 0;
+}
+
+extern "C" int cpfft_update(cpfft_ctx* c) {
+  std::swap(c->fields[CPFFT_STATE_N], c->fields[CPFFT_STATE_NP1]);
+  launch_copy(9 * c->V, c->fields[CPFFT_FN1], c->fields[CPFFT_FN], c->stream);
+  launch_copy(9 * c->V, c->fields[CPFFT_PN1], c->fields[CPFFT_PN], c->stream);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int cpfft_counters(cpfft_ctx* c, long long out[8]) {
+  long long d[8];
+  HIP_OK(hipMemcpy(d, c->dcnt, sizeof d, hipMemcpyDeviceToHost));
+  out[0] = c->gkdf_calls;
+  out[1] = c->cg_iters;
+  out[2] = c->des_calls;
+  out[3] = d[0];
+  out[4] = d[1];
+  out[5] = d[2];
+  out[6] = d[3];
+  out[7] = d[4];
+  return 0;
+}

@@ -1,0 +1,556 @@
+// cpfft_kernels.hip -- CDNA4 (gfx950) kernels of the CPFFT solver step.
+//
+//  * G_K_dF (src/G_K_dF.f:11-87) as five passes over an R2C half spectrum
+//    held in HBM, [9][N][N][N/2+1] complex (x slowest):
+//      1. z: fused K4:F contraction (ddot42n, G_K_dF.f:241-268) + real-to-
+//         complex FFT along z, one workgroup per (x,y) pencil
+//      2. y: complex FFT along y, one workgroup per (c, x, kz tile)
+//      3. x: forward FFT along x, Green projection for all nine
+//         components, inverse FFT along x, one workgroup per (ky, kz tile)
+//      4. y: inverse complex FFT
+//      5. z: complex-to-real FFT with the 1/N^3 scale
+//    FFTs run in LDS as mixed-radix (4,2,3,5,generic) Stockham stages with
+//    ping-pong buffers; twiddles come from a table of N-th roots of unity.
+//  * BLAS-1 and reductions for CG, Newton norms and volume averages
+//    (FFT_nr3.f:85-147, tangent_homo.f:62-66): deterministic two-stage
+//    block reductions (fixed order, no floating-point atomics).
+//  * drive_eps_sig (src/drive_eps_sig.f:16-339): one thread per voxel,
+//    body in mm10_core.h.
+#include "kernels.h"
+
+namespace cpfft {
+
+static constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ FFT
+// Twiddle for exponent e (mod N) of the N-th root table, conjugated for
+// the inverse transform.
+__device__ inline cplx tw_get(const cplx* __restrict__ tw, int N, long e, double sgn) {
+  int k = (int)(e % N);
+  cplx w = tw[k];
+  w.im *= sgn;  // table holds exp(-2 pi i k/N); sgn=+1 forward, -1 inverse
+  return w;
+}
+
+__device__ inline cplx cmul(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// Batched in-LDS complex FFT.  Line L, element n lives at
+// buf[L*ls + n*es]; tmp has the same layout.  fwd: +1 forward, -1 inverse
+// (unscaled).  Result in buf.  All threads of the block must call it.
+__device__ void lds_fft(cplx* buf, cplx* tmp, int nlines, int ls, int es, const FftPlan& pl,
+                        const cplx* __restrict__ tw, double fwd) {
+  const int N = pl.n;
+  const int T = blockDim.x, tid = threadIdx.x;
+  int Ns = 1;
+  cplx* src = buf;
+  cplx* dst = tmp;
+  for (int st = 0; st < pl.nst; ++st) {
+    const int r = pl.rad[st];
+    const int nb = N / r;
+    const int total = nlines * nb;
+    const int tstride = N / (Ns * r);  // twiddle exponent step: exp(-2pi i t k/(Ns r))
+    for (int b = tid; b < total; b += T) {
+      const int line = b / nb;
+      const int j = b - line * nb;
+      const int k = j % Ns;
+      const cplx* s = src + line * ls;
+      cplx* d = dst + line * ls;
+      const int o = (j / Ns) * Ns * r + k;
+      if (r == 2) {
+        cplx a0 = s[j * es];
+        cplx a1 = cmul(s[(j + nb) * es], tw_get(tw, N, (long)k * tstride, fwd));
+        d[o * es] = cplx{a0.re + a1.re, a0.im + a1.im};
+        d[(o + Ns) * es] = cplx{a0.re - a1.re, a0.im - a1.im};
+      } else if (r == 4) {
+        cplx a0 = s[j * es];
+        cplx a1 = cmul(s[(j + nb) * es], tw_get(tw, N, (long)k * tstride, fwd));
+        cplx a2 = cmul(s[(j + 2 * nb) * es], tw_get(tw, N, (long)2 * k * tstride, fwd));
+        cplx a3 = cmul(s[(j + 3 * nb) * es], tw_get(tw, N, (long)3 * k * tstride, fwd));
+        cplx b0{a0.re + a2.re, a0.im + a2.im}, b1{a0.re - a2.re, a0.im - a2.im};
+        cplx b2{a1.re + a3.re, a1.im + a3.im}, b3{a1.re - a3.re, a1.im - a3.im};
+        // multiply b3 by -i (forward) or +i (inverse)
+        cplx b3r{fwd * b3.im, -fwd * b3.re};
+        d[o * es] = cplx{b0.re + b2.re, b0.im + b2.im};
+        d[(o + Ns) * es] = cplx{b1.re + b3r.re, b1.im + b3r.im};
+        d[(o + 2 * Ns) * es] = cplx{b0.re - b2.re, b0.im - b2.im};
+        d[(o + 3 * Ns) * es] = cplx{b1.re - b3r.re, b1.im - b3r.im};
+      } else {
+        // generic radix r (3, 5, or a prime factor): direct DFT of size r
+        cplx x[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          if (t < r) {
+            cplx v = s[(j + t * nb) * es];
+            x[t] = t ? cmul(v, tw_get(tw, N, (long)t * k * tstride, fwd)) : v;
+          }
+        }
+        const int rstep = N / r;  // r-th roots from the N-th root table
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          if (q < r) {
+            cplx acc = x[0];
+#pragma unroll
+            for (int t = 1; t < 16; ++t) {
+              if (t < r) {
+                cplx w = tw_get(tw, N, (long)((q * t) % r) * rstep, fwd);
+                cplx m = cmul(x[t], w);
+                acc.re += m.re;
+                acc.im += m.im;
+              }
+            }
+            d[(o + q * Ns) * es] = acc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    cplx* t2 = src;
+    src = dst;
+    dst = t2;
+    Ns *= r;
+  }
+  if (src != buf) {
+    for (int b = tid; b < nlines * N; b += T) {
+      int line = b / N, n = b - line * N;
+      buf[line * ls + n * es] = src[line * ls + n * es];
+    }
+    __syncthreads();
+  }
+}
+
+// pass 1: (x,y) pencils. out spec[c][x][y][kz]
+__global__ __launch_bounds__(kBlock) void k_fft_z_fwd(SpecGeom g, const double* __restrict__ F,
+                                                      const double* __restrict__ K4,
+                                                      cplx* __restrict__ spec,
+                                                      const cplx* __restrict__ tw) {
+  extern __shared__ cplx lds[];
+  const int N = g.N, Nz = g.Nz;
+  const long V = g.V;
+  const long line0 = blockIdx.x;  // x*N + y
+  cplx* buf = lds;
+  cplx* tmp = lds + 9 * N;
+  const long e0 = line0 * N;
+  for (int z = threadIdx.x; z < N; z += blockDim.x) {
+    const long e = e0 + z;
+    double f[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) f[c] = F[c * V + e];
+    if (K4) {
+      // ddot42n: C_r = t1 + (((t2+t6)+(t4+t8)) + ((t3+t7)+(t5+t9)))
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        double t[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = K4[(9 * r + c) * V + e] * f[c];
+        double v = t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
+        buf[r * N + z] = cplx{v, 0.0};
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) buf[r * N + z] = cplx{f[r], 0.0};
+    }
+  }
+  __syncthreads();
+  lds_fft(buf, tmp, 9, N, 1, g.plan, tw, 1.0);
+  for (int i = threadIdx.x; i < 9 * Nz; i += blockDim.x) {
+    int c = i / Nz, kz = i - c * Nz;
+    spec[c * g.S1 + line0 * Nz + kz] = buf[c * N + kz];
+  }
+}
+
+// pass 2 / 4: y transforms. grid: (ceil(Nz/ty), N, 9)
+__global__ __launch_bounds__(kBlock) void k_fft_y(SpecGeom g, cplx* __restrict__ spec,
+                                                  const cplx* __restrict__ tw, double fwd) {
+  extern __shared__ cplx lds[];
+  const int N = g.N, Nz = g.Nz, ty = g.ty;
+  const int kz0 = blockIdx.x * ty;
+  const int x = blockIdx.y, c = blockIdx.z;
+  const int nt = min(ty, Nz - kz0);
+  cplx* base = spec + c * g.S1 + (long)x * N * Nz;
+  cplx* buf = lds;
+  cplx* tmp = lds + ty * N;
+  // line t = kz column, element n = y; layout buf[t*N + n]
+  for (int i = threadIdx.x; i < N * nt; i += blockDim.x) {
+    int y = i / nt, t = i - y * nt;
+    buf[t * N + y] = base[(long)y * Nz + kz0 + t];
+  }
+  __syncthreads();
+  lds_fft(buf, tmp, nt, N, 1, g.plan, tw, fwd);
+  for (int i = threadIdx.x; i < N * nt; i += blockDim.x) {
+    int y = i / nt, t = i - y * nt;
+    base[(long)y * Nz + kz0 + t] = buf[t * N + y];
+  }
+}
+
+// Green operator on one frequency: out_ij = q_j (A q)_i / |q|^2, added with weight w
+__device__ inline void gamma_acc(const double q[3], const cplx A[9], cplx out[9], double w) {
+  double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  if (qq <= 1e-10) return;
+  double s = w / qq;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    cplx aq{A[3 * i].re * q[0] + A[3 * i + 1].re * q[1] + A[3 * i + 2].re * q[2],
+            A[3 * i].im * q[0] + A[3 * i + 1].im * q[1] + A[3 * i + 2].im * q[2]};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      out[3 * i + j].re += aq.re * q[j] * s;
+      out[3 * i + j].im += aq.im * q[j] * s;
+    }
+  }
+}
+
+__device__ inline int wrapf(int v, int N) {
+  // representative in [-N/2, N/2) for even N, (-N/2, N/2] for odd N
+  int h = N / 2;
+  int m = ((v + h) % N + N) % N;
+  return m - h;
+}
+
+// pass 3: x transforms + projection. grid: (ceil(Nz/tz), N)
+__global__ __launch_bounds__(kBlock) void k_proj_x(SpecGeom g, cplx* __restrict__ spec,
+                                                   const cplx* __restrict__ tw) {
+  extern __shared__ cplx lds[];
+  const int N = g.N, Nz = g.Nz, tz = g.tz;
+  const int kz0 = blockIdx.x * tz;
+  const int y = blockIdx.y;
+  const int nt = min(tz, Nz - kz0);
+  const int nl = 9 * nt;
+  cplx* buf = lds;
+  cplx* tmp = lds + 9 * tz * N;
+  // line (c*nt + t), element x; layout buf[(c*nt+t)*N + x]
+  for (int i = threadIdx.x; i < 9 * N * nt; i += blockDim.x) {
+    int t = i % nt;
+    int rest = i / nt;
+    int x = rest % N, c = rest / N;
+    buf[(c * nt + t) * N + x] = spec[c * g.S1 + ((long)x * N + y) * Nz + kz0 + t];
+  }
+  __syncthreads();
+  lds_fft(buf, tmp, nl, N, 1, g.plan, tw, 1.0);
+  // projection at every (x, kz) point of the tile
+  const bool even = (N % 2) == 0;
+  for (int i = threadIdx.x; i < N * nt; i += blockDim.x) {
+    int x = i / nt, t = i - x * nt;
+    int f[3] = {x <= N / 2 ? x : x - N, y <= N / 2 ? y : y - N, kz0 + t};
+    cplx A[9], out[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      A[c] = buf[(c * nt + t) * N + x];
+      out[c] = cplx{0.0, 0.0};
+    }
+    if (g.gamma_mode == 0 && even) {
+      // reference operator: M(f) = (Gamma(w(f-1)) + Gamma(w(-f-1)))/2
+      double q1[3], q2[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        q1[a] = (double)wrapf(f[a] - 1, N);
+        q2[a] = (double)wrapf(-f[a] - 1, N);
+      }
+      gamma_acc(q1, A, out, 0.5);
+      gamma_acc(q2, A, out, 0.5);
+    } else {
+      bool nyq = even && (abs(f[0]) == N / 2 || abs(f[1]) == N / 2 || f[2] == N / 2);
+      if (!nyq) {
+        double q[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) q[a] = (double)wrapf(f[a], N);
+        gamma_acc(q, A, out, 1.0);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) buf[(c * nt + t) * N + x] = out[c];
+  }
+  __syncthreads();
+  lds_fft(buf, tmp, nl, N, 1, g.plan, tw, -1.0);
+  for (int i = threadIdx.x; i < 9 * N * nt; i += blockDim.x) {
+    int t = i % nt;
+    int rest = i / nt;
+    int x = rest % N, c = rest / N;
+    spec[c * g.S1 + ((long)x * N + y) * Nz + kz0 + t] = buf[(c * nt + t) * N + x];
+  }
+}
+
+// pass 5: complex-to-real along z, scale 1/N^3
+__global__ __launch_bounds__(kBlock) void k_fft_z_inv(SpecGeom g, const cplx* __restrict__ spec,
+                                                      double* __restrict__ out,
+                                                      const cplx* __restrict__ tw) {
+  extern __shared__ cplx lds[];
+  const int N = g.N, Nz = g.Nz;
+  const long V = g.V;
+  const long line0 = blockIdx.x;
+  cplx* buf = lds;
+  cplx* tmp = lds + 9 * N;
+  for (int i = threadIdx.x; i < 9 * N; i += blockDim.x) {
+    int c = i / N, z = i - c * N;
+    cplx v;
+    if (z < Nz) {
+      v = spec[c * g.S1 + line0 * Nz + z];
+    } else {
+      cplx w = spec[c * g.S1 + line0 * Nz + (N - z)];
+      v = cplx{w.re, -w.im};
+    }
+    buf[c * N + z] = v;
+  }
+  __syncthreads();
+  lds_fft(buf, tmp, 9, N, 1, g.plan, tw, -1.0);
+  const double sc = 1.0 / (double)V;
+  for (int i = threadIdx.x; i < 9 * N; i += blockDim.x) {
+    int c = i / N, z = i - c * N;
+    out[c * V + line0 * N + z] = buf[c * N + z].re * sc;
+  }
+}
+
+void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
+                 const cplx* tw, hipStream_t s) {
+  const int N = g.N;
+  size_t lz = 2 * 9 * (size_t)N * sizeof(cplx);
+  k_fft_z_fwd<<<dim3((unsigned)((long)N * N)), kBlock, lz, s>>>(g, F, K4, spec, tw);
+  size_t ly = 2 * (size_t)g.ty * N * sizeof(cplx);
+  dim3 gy((g.Nz + g.ty - 1) / g.ty, N, 9);
+  k_fft_y<<<gy, kBlock, ly, s>>>(g, spec, tw, 1.0);
+  size_t lx = 2 * 9 * (size_t)g.tz * N * sizeof(cplx);
+  dim3 gx((g.Nz + g.tz - 1) / g.tz, N);
+  k_proj_x<<<gx, kBlock, lx, s>>>(g, spec, tw);
+  k_fft_y<<<gy, kBlock, ly, s>>>(g, spec, tw, -1.0);
+  k_fft_z_inv<<<dim3((unsigned)((long)N * N)), kBlock, lz, s>>>(g, spec, out, tw);
+}
+
+// ------------------------------------------------------------ reductions
+static constexpr int kRedBlocks = 1024;
+
+int reduce_partials_count(long) { return kRedBlocks * 9; }
+
+__device__ inline double block_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// fixed partition: block b handles the contiguous range [b*chunk, (b+1)*chunk)
+__global__ __launch_bounds__(kBlock) void k_dot_part(const double* __restrict__ x,
+                                                     const double* __restrict__ y, long n,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  long chunk = (n + gridDim.x - 1) / gridDim.x;
+  long a = blockIdx.x * chunk, b = min(n, a + chunk);
+  double acc = 0.0;
+  for (long i = a + threadIdx.x; i < b; i += blockDim.x) acc += x[i] * y[i];
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const double* __restrict__ part, int np,
+                                                      int nres, double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  for (int r = 0; r < nres; ++r) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[r * np + i];
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[r] = s;
+  }
+}
+
+void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,
+                hipStream_t s) {
+  k_dot_part<<<kRedBlocks, kBlock, 0, s>>>(x, y, n, partials);
+  k_sum_parts<<<1, kBlock, 0, s>>>(partials, kRedBlocks, 1, d_out);
+}
+
+// mean of each of the 9 components: partial sums per component
+__global__ __launch_bounds__(kBlock) void k_sum9_part(const double* __restrict__ x, long V,
+                                                      double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  long chunk = (V + gridDim.x - 1) / gridDim.x;
+  long a = blockIdx.x * chunk, b = min(V, a + chunk);
+  for (int c = 0; c < 9; ++c) {
+    double acc = 0.0;
+    for (long i = a + threadIdx.x; i < b; i += blockDim.x) acc += x[c * V + i];
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[c * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ void k_scale9(double* out, double sc) {
+  if (threadIdx.x < 9) out[threadIdx.x] *= sc;
+}
+
+void launch_mean9(const double* x, long V, double* partials, double* d_out, hipStream_t s) {
+  k_sum9_part<<<kRedBlocks, kBlock, 0, s>>>(x, V, partials);
+  k_sum_parts<<<1, kBlock, 0, s>>>(partials, kRedBlocks, 9, d_out);
+  k_scale9<<<1, 64, 0, s>>>(d_out, 1.0 / (double)V);
+}
+
+// tangent_homo: out[j] = sum over (m,e) of K4[9j+m][e] C[m][e]
+__global__ __launch_bounds__(kBlock) void k_k4dot9_part(const double* __restrict__ K4,
+                                                        const double* __restrict__ C, long V,
+                                                        double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  long n = 9 * V;
+  long chunk = (n + gridDim.x - 1) / gridDim.x;
+  long a = blockIdx.x * chunk, b = min(n, a + chunk);
+  for (int j = 0; j < 9; ++j) {
+    const double* Kj = K4 + (long)9 * j * V;
+    double acc = 0.0;
+    for (long i = a + threadIdx.x; i < b; i += blockDim.x) acc += Kj[i] * C[i];
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[j * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+void launch_k4_dot9(long V, const double* K4, const double* C, double* partials, double* d_out,
+                    hipStream_t s) {
+  k_k4dot9_part<<<kRedBlocks, kBlock, 0, s>>>(K4, C, V, partials);
+  k_sum_parts<<<1, kBlock, 0, s>>>(partials, kRedBlocks, 9, d_out);
+  k_scale9<<<1, 64, 0, s>>>(d_out, 1.0 / (double)V);
+}
+
+// --------------------------------------------------------------- BLAS-1
+__global__ void k_axpy(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = y[i] + a * x[i];
+}
+__global__ void k_scal(long n, double a, double* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = a * x[i];
+}
+__global__ void k_copy(long n, const double* __restrict__ x, double* __restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+struct V9 {
+  double v[9];
+};
+__global__ void k_fill9(long V, V9 v, double* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < 9 * V; i += (long)gridDim.x * blockDim.x)
+    x[i] = v.v[i / V];
+}
+__global__ void k_add1(long V, double* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < V; i += (long)gridDim.x * blockDim.x)
+    x[i] = x[i] + 1.0;
+}
+__global__ void k_gather_col(long V, int col, const double* __restrict__ K4, double* __restrict__ C) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < 9 * V; i += (long)gridDim.x * blockDim.x) {
+    long j = i / V, e = i - j * V;
+    C[i] = K4[(9 * j + col) * V + e];
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_cg_update(long n, double a, const double* __restrict__ p,
+                                                      const double* __restrict__ q,
+                                                      double* __restrict__ x, double* __restrict__ r,
+                                                      double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  long chunk = (n + gridDim.x - 1) / gridDim.x;
+  long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double acc = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    x[i] = x[i] + a * p[i];
+    double ri = r[i] - a * q[i];
+    r[i] = ri;
+    acc += ri * ri;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void k_cg_dir(long n, double b, const double* __restrict__ r, double* __restrict__ p) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = r[i] + b * p[i];
+}
+
+static inline unsigned grid_for(long n) {
+  long g = (n + kBlock - 1) / kBlock;
+  return (unsigned)(g > 8192 ? 8192 : (g < 1 ? 1 : g));
+}
+
+void launch_axpy(long n, double a, const double* x, double* y, hipStream_t s) {
+  k_axpy<<<grid_for(n), kBlock, 0, s>>>(n, a, x, y);
+}
+void launch_scal(long n, double a, double* x, hipStream_t s) {
+  k_scal<<<grid_for(n), kBlock, 0, s>>>(n, a, x);
+}
+void launch_copy(long n, const double* x, double* y, hipStream_t s) {
+  k_copy<<<grid_for(n), kBlock, 0, s>>>(n, x, y);
+}
+void launch_fill9(long V, const double* v9, double* x, hipStream_t s) {
+  V9 v;
+  for (int i = 0; i < 9; ++i) v.v[i] = v9[i];
+  k_fill9<<<grid_for(9 * V), kBlock, 0, s>>>(V, v, x);
+}
+void launch_add_identity_comp(long V, int comp, double* x, hipStream_t s) {
+  k_add1<<<grid_for(V), kBlock, 0, s>>>(V, x + (long)comp * V);
+}
+void launch_gather_k4_col(long V, int i, const double* K4, double* C, hipStream_t s) {
+  k_gather_col<<<grid_for(9 * V), kBlock, 0, s>>>(V, i, K4, C);
+}
+void launch_cg_update(long n, double a, const double* p, const double* q, double* x, double* r,
+                      double* partials, double* d_out, hipStream_t s) {
+  k_cg_update<<<kRedBlocks, kBlock, 0, s>>>(n, a, p, q, x, r, partials);
+  k_sum_parts<<<1, kBlock, 0, s>>>(partials, kRedBlocks, 1, d_out);
+}
+void launch_cg_dir(long n, double b, const double* r, double* p, hipStream_t s) {
+  k_cg_dir<<<grid_for(n), kBlock, 0, s>>>(n, b, r, p);
+}
+
+// ------------------------------------------------------- stress update
+template <int NS>
+__global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
+                                                      const Orient<NS>* __restrict__ orients,
+                                                      const int32_t* __restrict__ oid, int step,
+                                                      int iter, const double* __restrict__ Fn,
+                                                      const double* __restrict__ Fn1,
+                                                      const double* __restrict__ sn,
+                                                      double* __restrict__ snp1,
+                                                      double* __restrict__ Pn1,
+                                                      double* __restrict__ K4,
+                                                      long long* __restrict__ counters) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  Counters cnt{0, 0, 0, 0, 0};
+  if (e < V) {
+    VoxelIO io;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        io.Fn[i][j] = Fn[(3 * i + j) * V + e];
+        io.Fn1[i][j] = Fn1[(3 * i + j) * V + e];
+      }
+    const Orient<NS>& o = orients[oid[e]];
+    voxel_update<NS>(
+        P, o, step, iter, io, [&](int w) { return sn[w * V + e]; },
+        [&](int w, double v) { snp1[w * V + e] = v; },
+        [&](int c, double v) { Pn1[c * V + e] = v; },
+        [&](int c, double v) { K4[c * V + e] = v; }, cnt);
+  }
+  // wavefront-level sums of the counters, one integer atomic per wavefront
+  long long c[5] = {cnt.pred_it, cnt.upd_it, cnt.cuts, cnt.failed, cnt.ls_evals};
+  for (int k = 0; k < 5; ++k) {
+    long long v = c[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)&counters[k], (unsigned long long)v);
+  }
+}
+
+template <int NS>
+void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
+                          const int32_t* orient_id, int step, int iter, const double* Fn,
+                          const double* Fn1, const double* sn, double* snp1, double* Pn1,
+                          double* K4, long long* counters, hipStream_t s) {
+  const int bs = 64;
+  unsigned g = (unsigned)((V + bs - 1) / bs);
+  k_drive_eps_sig<NS><<<g, bs, 0, s>>>(V, P, orients, orient_id, step, iter, Fn, Fn1, sn, snp1,
+                                       Pn1, K4, counters);
+}
+
+template void launch_drive_eps_sig<12>(long, const MatParams&, const Orient<12>*, const int32_t*,
+                                       int, int, const double*, const double*, const double*,
+                                       double*, double*, double*, long long*, hipStream_t);
+template void launch_drive_eps_sig<24>(long, const MatParams&, const Orient<24>*, const int32_t*,
+                                       int, int, const double*, const double*, const double*,
+                                       double*, double*, double*, long long*, hipStream_t);
+
+}  // namespace cpfft

@@ -1,0 +1,57 @@
+// kernels.h -- launch wrappers for the CDNA4 kernels (defined in
+// cpfft_kernels.hip), called by the context code in cpfft_api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cpfft_common.h"
+#include "mm10_core.h"
+
+namespace cpfft {
+
+struct SpecGeom {
+  int N, Nz;        // grid edge, number of z frequencies kept (N/2+1)
+  long V;           // N^3
+  long S1;          // complex entries per spectral component: N*N*Nz
+  FftPlan plan;     // radix plan for length N
+  int gamma_mode;   // 0 reference operator, 1 textbook projection
+  int ty;           // kz columns per block in the y pass
+  int tz;           // kz columns per block in the x pass
+};
+
+// Gamma:(K4:F) or Gamma:F.  spec: 9*S1 complex scratch.  tw: N twiddles.
+// If dot_with != nullptr, also accumulates sum(out * dot_with) partials into
+// partials (one per z-line block) for a fused inner product.
+void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
+                 const cplx* tw, hipStream_t s);
+
+// reductions: result -> d_out[0..nres) on device (two-stage, fixed order)
+void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,
+                hipStream_t s);
+void launch_mean9(const double* x, long V, double* partials, double* d_out, hipStream_t s);
+void launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);
+void launch_scal(long n, double a, double* x, hipStream_t s);
+void launch_copy(long n, const double* x, double* y, hipStream_t s);
+void launch_fill9(long V, const double* v9_host, double* x, hipStream_t s);
+void launch_add_identity_comp(long V, int comp, double* x, hipStream_t s);
+// CG vector update: x += a p; r -= a q; partial r.r into partials
+void launch_cg_update(long n, double a, const double* p, const double* q, double* x, double* r,
+                      double* partials, double* d_out, hipStream_t s);
+// p = r + b p
+void launch_cg_dir(long n, double b, const double* r, double* p, hipStream_t s);
+// C[j] = K4[(9 j + i) rows] copy (tangent_homo column i)
+void launch_gather_k4_col(long V, int i, const double* K4, double* C, hipStream_t s);
+// out[j] = sum_e sum_m K4[9j+m][e] * C[m][e] for j=0..8 (tangent_homo.f:62-66)
+void launch_k4_dot9(long V, const double* K4, const double* C, double* partials, double* d_out,
+                    hipStream_t s);
+
+// per-voxel update
+template <int NS>
+void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
+                          const int32_t* orient_id, int step, int iter, const double* Fn,
+                          const double* Fn1, const double* sn, double* snp1, double* Pn1,
+                          double* K4, long long* counters, hipStream_t s);
+
+int reduce_partials_count(long n);
+
+}  // namespace cpfft

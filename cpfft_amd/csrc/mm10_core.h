@@ -1,0 +1,939 @@
+// mm10_core.h -- per-voxel finite-strain crystal-plasticity stress update
+// (kinematics + mm10 Voce model + consistent tangent + exact dP/dF), the
+// body of the drive_eps_sig kernel.  One thread updates one voxel; all
+// per-voxel temporaries live in registers.
+//
+// Reference routines (src/): do_nleps_block drive_eps_sig.f:57-339, polar
+// decomposition polar.f:18-307, getrm1 polar.f:680-880, qmply1.f:15-49,
+// mm10 mm10_a.f:29-355, mm10_setup mm10_a.f:830-962, mm10_solve_strup
+// mm10_a.f:2628-2845, mm10_solve mm10_a.f:2860-3295, mm10_tangent
+// mm10_a.f:658-815, mm10_update_rotation mm10_a.f:3310-3415, mm10_output
+// mm10_a.f:3429-3689, residual/Jacobian/Voce law mm10_b.f:63-2071,
+// cep2A cep2A.f:86-284, cs2p drive_eps_sig.f:1182-1224.
+//
+// Differences in arithmetic organisation from the reference (results agree
+// to rounding): the power law is evaluated once per slip system and
+// evaluation point, by repeated squaring when the rate exponent is an
+// integer; the Jacobian used by the tangent is re-formed from the saved
+// iterate instead of being kept across the line search; dP/dF is assembled
+// one (k,l) column at a time from 3x3 blocks.  The polar decomposition is
+// evaluated with the reference's operation order and without FMA
+// contraction because its Cardano eigenvalue step amplifies rounding.
+#pragma once
+
+#include "cpfft_common.h"
+
+namespace cpfft {
+
+// ---------------------------------------------------------------- params
+struct MatParams {
+  double rate_n, tau_y, tau_v, theta_0, voche_m, eps_dot_0_y, dt;
+  double atol, atol1, rtol, rtol1;
+  int miter, alter_mode, jac_exact, int_rate;  // int_rate: rate_n integral (>=2)
+  int m_is_one;                                // voche_m == 1
+};
+
+// per-orientation invariants (setup_mm10_rknstr, drive_eps_sig.f:537-1002):
+// rotated stiffness C (row-major 6x6), ms (Voigt 6, eng. shear), qs (skew 3)
+template <int NS>
+struct Orient {
+  double C[36];
+  double ms[NS][6];
+  double qs[NS][3];
+};
+
+struct Counters {
+  int pred_it, upd_it, cuts, failed, ls_evals;
+};
+
+// ---------------------------------------------------------------- helpers
+CPFFT_HD inline double sgn1(double x) { return copysign(1.0, x); }
+
+CPFFT_HD inline double ipow(double x, int n) {
+  double r = 1.0, b = x;
+  while (n > 0) {
+    if (n & 1) r *= b;
+    b *= b;
+    n >>= 1;
+  }
+  return r;
+}
+
+// |u|^(n-2) for the power law (n = rate exponent)
+CPFFT_HD inline double pw_nm2(const MatParams& p, double au) {
+  return p.int_rate ? ipow(au, (int)p.rate_n - 2) : pow(au, p.rate_n - 2.0);
+}
+
+// LU with partial pivoting + solve, n <= 7, row-major A (destroyed)
+template <int N, int NR>
+CPFFT_HD inline void lu_solve(double (&A)[N][N], double (&B)[N][NR]) {
+  int piv[N];
+  for (int j = 0; j < N; ++j) {
+    int p = j;
+    double amax = fabs(A[j][j]);
+    for (int i = j + 1; i < N; ++i) {
+      double v = fabs(A[i][j]);
+      if (v > amax) { amax = v; p = i; }
+    }
+    piv[j] = p;
+    if (p != j)
+      for (int k = 0; k < N; ++k) { double t = A[j][k]; A[j][k] = A[p][k]; A[p][k] = t; }
+    double rp = 1.0 / A[j][j];
+    for (int i = j + 1; i < N; ++i) A[i][j] *= rp;
+    for (int i = j + 1; i < N; ++i) {
+      double l = A[i][j];
+      for (int k = j + 1; k < N; ++k) A[i][k] -= l * A[j][k];
+    }
+  }
+  for (int j = 0; j < N; ++j)
+    if (piv[j] != j)
+      for (int r = 0; r < NR; ++r) { double t = B[j][r]; B[j][r] = B[piv[j]][r]; B[piv[j]][r] = t; }
+  for (int r = 0; r < NR; ++r) {
+    for (int i = 1; i < N; ++i) {
+      double s = B[i][r];
+      for (int k = 0; k < i; ++k) s -= A[i][k] * B[k][r];
+      B[i][r] = s;
+    }
+    for (int i = N - 1; i >= 0; --i) {
+      double s = B[i][r];
+      for (int k = i + 1; k < N; ++k) s -= A[i][k] * B[k][r];
+      B[i][r] = s / A[i][i];
+    }
+  }
+}
+
+CPFFT_HD inline void rt2rve(const double rt[3][3], double rv[6][6]) {
+  for (int r = 0; r < 3; ++r) {
+    rv[r][0] = rt[r][0] * rt[r][0];
+    rv[r][1] = rt[r][1] * rt[r][1];
+    rv[r][2] = rt[r][2] * rt[r][2];
+    rv[r][3] = 2.0 * rt[r][0] * rt[r][1];
+    rv[r][4] = 2.0 * rt[r][2] * rt[r][1];
+    rv[r][5] = 2.0 * rt[r][0] * rt[r][2];
+  }
+  const int a[3] = {0, 1, 0}, b[3] = {1, 2, 2};
+  for (int q = 0; q < 3; ++q) {
+    int i = a[q], j = b[q];
+    rv[3 + q][0] = rt[i][0] * rt[j][0];
+    rv[3 + q][1] = rt[i][1] * rt[j][1];
+    rv[3 + q][2] = rt[i][2] * rt[j][2];
+    rv[3 + q][3] = rt[i][0] * rt[j][1] + rt[j][0] * rt[i][1];
+    rv[3 + q][4] = rt[i][1] * rt[j][2] + rt[i][2] * rt[j][1];
+    rv[3 + q][5] = rt[i][0] * rt[j][2] + rt[i][2] * rt[j][0];
+  }
+}
+
+CPFFT_HD inline void rt2rvw(const double rt[3][3], double rv[3][3]) {
+  rv[0][0] = rt[1][1] * rt[2][2] - rt[1][2] * rt[2][1];
+  rv[0][1] = rt[1][0] * rt[2][2] - rt[1][2] * rt[2][0];
+  rv[0][2] = rt[1][0] * rt[2][1] - rt[1][1] * rt[2][0];
+  rv[1][0] = rt[0][1] * rt[2][2] - rt[0][2] * rt[2][1];
+  rv[1][1] = rt[0][0] * rt[2][2] - rt[0][2] * rt[2][0];
+  rv[1][2] = rt[0][0] * rt[2][1] - rt[0][1] * rt[2][0];
+  rv[2][0] = rt[0][1] * rt[1][2] - rt[0][2] * rt[1][1];
+  rv[2][1] = rt[0][0] * rt[1][2] - rt[0][2] * rt[1][0];
+  rv[2][2] = rt[0][0] * rt[1][1] - rt[0][1] * rt[1][0];
+}
+
+// mm10_symsw (mm10_b.f:1505-1533)
+CPFFT_HD inline void symsw(const double s[6], const double w[3], double sw[6]) {
+  sw[0] = s[3] * w[2] - s[5] * w[1];
+  sw[1] = s[3] * w[2] - s[4] * w[0];
+  sw[2] = s[5] * w[1] + s[4] * w[0];
+  sw[3] = 0.5 * (w[2] * (s[0] - s[1]) + w[0] * s[5] - w[1] * s[4]);
+  sw[4] = 0.5 * (w[0] * (s[1] - s[2]) + w[1] * s[3] + w[2] * s[5]);
+  sw[5] = 0.5 * (w[1] * (s[0] - s[2]) + w[0] * s[3] - w[2] * s[4]);
+}
+
+// ---------------------------------------------------------- kinematics
+// Polar decomposition R = F U^-1 with Cardano eigenvalues (polar.f:18-307).
+// Operation order follows the reference; no FMA contraction.
+CPFFT_HD inline void polar_R(const double f[3][3], double r[3][3]) {
+#pragma clang fp contract(off)
+  double c[6], cc[6];
+  c[0] = f[0][0] * f[0][0] + f[1][0] * f[1][0] + f[2][0] * f[2][0];
+  c[1] = f[0][0] * f[0][1] + f[1][0] * f[1][1] + f[2][0] * f[2][1];
+  c[2] = f[0][1] * f[0][1] + f[1][1] * f[1][1] + f[2][1] * f[2][1];
+  c[3] = f[0][0] * f[0][2] + f[1][0] * f[1][2] + f[2][0] * f[2][2];
+  c[4] = f[0][1] * f[0][2] + f[1][1] * f[1][2] + f[2][1] * f[2][2];
+  c[5] = f[0][2] * f[0][2] + f[1][2] * f[1][2] + f[2][2] * f[2][2];
+  cc[0] = c[0] * c[0] + c[1] * c[1] + c[3] * c[3];
+  cc[1] = c[0] * c[1] + c[1] * c[2] + c[3] * c[4];
+  cc[2] = c[1] * c[1] + c[2] * c[2] + c[4] * c[4];
+  cc[3] = c[0] * c[3] + c[1] * c[4] + c[3] * c[5];
+  cc[4] = c[1] * c[3] + c[2] * c[4] + c[4] * c[5];
+  cc[5] = c[3] * c[3] + c[4] * c[4] + c[5] * c[5];
+  const double third = 0.3333333333333333333, oneroot3 = 0.5773502691896258;
+  double m11 = c[0], m12 = c[1], m13 = c[3], m22 = c[2], m23 = c[4], m33 = c[5];
+  double de = m12 * m23, dd = m12 * m12, ee = m23 * m23, ff = m13 * m13;
+  double m = m11 + m22 + m33;
+  double c1 = (m11 * m22 + m11 * m33 + m22 * m33) - (dd + ee + ff);
+  double c0 = m33 * dd + m11 * ee + m22 * ff - m11 * m22 * m33 - 2.0 * m13 * de;
+  double p = m * m - 3.0 * c1;
+  double q = m * (p - 1.5 * c1) - 13.5 * c0;
+  double sqrtp = sqrt(fabs(p));
+  double phi = 27.0 * (0.25 * c1 * c1 * (p - c1) + c0 * (q + 6.75 * c0));
+  phi = third * atan2(sqrt(fabs(phi)), q);
+  double cphi = sqrtp * cos(phi);
+  double sphi = oneroot3 * sqrtp * sin(phi);
+  double e2 = third * (m - cphi);
+  double e3 = e2 + sphi;
+  double e1 = e2 + cphi;
+  e2 = e2 - sphi;
+  double t;
+  if (e2 < e1) { t = e1; e1 = e2; e2 = t; }
+  if (e3 < e1) { t = e1; e1 = e3; e3 = t; }
+  if (e3 < e2) { t = e2; e2 = e3; e3 = t; }
+  e1 = sqrt(e1);
+  e2 = sqrt(e2);
+  e3 = sqrt(e3);
+  double iu = e1 + e2 + e3;
+  double iiu = e1 * e2 + e2 * e3 + e1 * e3;
+  double iiiu = e1 * e2 * e3;
+  double a2 = 1.0 / (iiiu * (iu * iiu - iiiu));
+  double b2 = iu * iiu * iiu - iiiu * (iu * iu + iiu);
+  double c2 = -iiiu - iu * (iu * iu - 2.0 * iiu);
+  double d2 = iu;
+  double ui[6];
+  ui[0] = a2 * (b2 + c2 * c[0] + d2 * cc[0]);
+  ui[1] = a2 * (c2 * c[1] + d2 * cc[1]);
+  ui[2] = a2 * (b2 + c2 * c[2] + d2 * cc[2]);
+  ui[3] = a2 * (c2 * c[3] + d2 * cc[3]);
+  ui[4] = a2 * (c2 * c[4] + d2 * cc[4]);
+  ui[5] = a2 * (b2 + c2 * c[5] + d2 * cc[5]);
+  for (int i = 0; i < 3; ++i) {
+    r[i][0] = f[i][0] * ui[0] + f[i][1] * ui[1] + f[i][2] * ui[3];
+    r[i][1] = f[i][0] * ui[1] + f[i][1] * ui[2] + f[i][2] * ui[4];
+    r[i][2] = f[i][0] * ui[3] + f[i][1] * ui[4] + f[i][2] * ui[5];
+  }
+}
+
+// inv33 (drive_eps_sig.f:1017-1107)
+CPFFT_HD inline double inv33(const double a[3][3], double g[3][3]) {
+#pragma clang fp contract(off)
+  double j1 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+  double j2 = a[1][0] * a[2][2] - a[1][2] * a[2][0];
+  double j3 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  double d = a[0][0] * j1 - a[0][1] * j2 + a[0][2] * j3;
+  g[0][0] = j1 / d;
+  g[1][0] = -j2 / d;
+  g[2][0] = j3 / d;
+  g[0][1] = (a[2][1] * a[0][2] - a[0][1] * a[2][2]) / d;
+  g[1][1] = (a[0][0] * a[2][2] - a[2][0] * a[0][2]) / d;
+  g[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / d;
+  g[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / d;
+  g[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / d;
+  g[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / d;
+  return d;
+}
+
+// ---------------------------------------------------------- material
+template <int NS>
+struct Point {        // crystal_state on the path (mm10_defs, mod_crystals.f:63-80)
+  double D[6];        // unrotated strain increment
+  double dg, tinc;    // slip-rate pre-factor, time increment
+  double ms[NS][6];   // current Schmid tensors (rotated by Rp_n^T)
+  double qc[NS][3];   // current spin vectors (rotated by R Rp_n^T)
+};
+
+template <int NS>
+struct StateN {
+  double stress[6];
+  double Rp[3][3];
+  double tt, tt_rate, D[6];
+};
+
+// slip-system quantities at (stress, tt): u = rs/tt, p = |u|^(n-2)
+template <int NS>
+struct SlipEval {
+  double rs[NS], slip[NS], pn2[NS];
+};
+
+template <int NS>
+CPFFT_HD inline void eval_slip(const MatParams& P, const Point<NS>& pt, const double s[6],
+                               double tt, SlipEval<NS>& ev) {
+  for (int i = 0; i < NS; ++i) {
+    double rs = s[0] * pt.ms[i][0] + s[1] * pt.ms[i][1] + s[2] * pt.ms[i][2] +
+                s[3] * pt.ms[i][3] + s[4] * pt.ms[i][4] + s[5] * pt.ms[i][5];
+    double u = rs / tt;
+    double au = fabs(u);
+    double p = pw_nm2(P, au);
+    ev.rs[i] = rs;
+    ev.pn2[i] = p;
+    ev.slip[i] = pt.dg * (p * au) * u;  // dg/tt |rs/tt|^(n-1) rs
+  }
+}
+
+// R1 (mm10_formR1, mm10_b.f:1065-1113) from slip increments
+template <int NS>
+CPFFT_HD inline void form_R1(const double* __restrict__ C, const Point<NS>& pt,
+                             const StateN<NS>& n, const double s[6], const SlipEval<NS>& ev,
+                             double R1[6], double wp[3]) {
+  double dbar[6] = {0, 0, 0, 0, 0, 0};
+  wp[0] = wp[1] = wp[2] = 0.0;
+  for (int i = 0; i < NS; ++i) {
+    double g = ev.slip[i];
+    for (int k = 0; k < 6; ++k) dbar[k] += g * pt.ms[i][k];
+    for (int k = 0; k < 3; ++k) wp[k] += g * pt.qc[i][k];
+  }
+  double sw[6], w1[6];
+  symsw(s, wp, sw);
+  for (int k = 0; k < 6; ++k) w1[k] = pt.D[k] - dbar[k];
+  for (int a = 0; a < 6; ++a) {
+    double c = 0.0;
+    for (int b = 0; b < 6; ++b) c += C[a * 6 + b] * w1[b];
+    R1[a] = s[a] - n.stress[a] - c + 2.0 * sw[a];
+  }
+}
+
+// h of the Voce law (mm10_h_voche, mm10_b.f:1888-1916), tau_l = 0
+template <int NS>
+CPFFT_HD inline double voce_h(const MatParams& P, const StateN<NS>& n, double tt,
+                              const SlipEval<NS>& ev) {
+  double h_term = 1.0 - (tt - P.tau_y) / P.tau_v;
+  double f = P.m_is_one ? h_term : pow(fabs(h_term), P.voche_m) * sgn1(h_term);
+  double s = 0.0;
+  for (int i = 0; i < NS; ++i) s += fabs(ev.slip[i]);
+  return n.tt + P.theta_0 * (f * s);
+}
+
+// full residual R (mm10_formR, mm10_b.f:1029-1063); returns |R|^2 and tt_rate
+template <int NS>
+CPFFT_HD inline double form_R(const MatParams& P, const double* __restrict__ C,
+                              const Point<NS>& pt, const StateN<NS>& n, const double x[7],
+                              double R[7], double* tt_rate) {
+  SlipEval<NS> ev;
+  eval_slip(P, pt, x, x[6], ev);
+  double wp[3];
+  form_R1(C, pt, n, x, ev, R, wp);
+  double h = voce_h(P, n, x[6], ev);
+  R[6] = x[6] - h;
+  *tt_rate = (h - n.tt) / pt.tinc;
+  double d = 0.0;
+  for (int k = 0; k < 7; ++k) d += R[k] * R[k];
+  return d;
+}
+
+// Jacobian (mm10_formJ11/J12/J21/J22, mm10_b.f:177-481, 1918-1983).
+// NJ = 6: stress block only (predictor); NJ = 7: full system.
+template <int NS, int NJ>
+CPFFT_HD inline void form_J(const MatParams& P, const double* __restrict__ C,
+                            const Point<NS>& pt, const double s[6], double tt,
+                            double (&J)[NJ][NJ]) {
+  SlipEval<NS> ev;
+  eval_slip(P, pt, s, tt, ev);
+  for (int a = 0; a < NJ; ++a)
+    for (int b = 0; b < NJ; ++b) J[a][b] = 0.0;
+  const double n_ = P.rate_n;
+  double h_term = 1.0 - (tt - P.tau_y) / P.tau_v;
+  double fh = P.m_is_one ? h_term : pow(fabs(h_term), P.voche_m) * sgn1(h_term);
+  double absh_m = P.m_is_one ? fabs(h_term) : pow(fabs(h_term), P.voche_m);
+  double wp[3] = {0, 0, 0};
+  double etau = 0.0;
+  for (int i = 0; i < NS; ++i) {
+    double sw[6], wv[6];
+    symsw(s, pt.qc[i], sw);
+    for (int a = 0; a < 6; ++a) {
+      double c = 2.0 * sw[a];
+      for (int b = 0; b < 6; ++b) c += C[a * 6 + b] * pt.ms[i][b];
+      wv[a] = c;
+    }
+    double au = fabs(ev.rs[i] / tt);
+    double dgdt = pt.dg * n_ / tt * (ev.pn2[i] * au);  // dg n/tt^n |rs|^(n-1)
+    for (int a = 0; a < 6; ++a) {
+      double t = dgdt * wv[a];
+      for (int b = 0; b < 6; ++b) J[a][b] += t * pt.ms[i][b];
+    }
+    for (int k = 0; k < 3; ++k) wp[k] += ev.slip[i] * pt.qc[i][k];
+    if (NJ == 7) {
+      double dgdtt = -n_ / tt * ev.slip[i];
+      for (int a = 0; a < 6; ++a) J[a][NJ - 1] += wv[a] * dgdtt;
+      // J21 = -dh/dsigma
+      double u = ev.rs[i] / tt;
+      double f21 = fh * ev.pn2[i] * u;  // |h|^m sgn(h) |u|^(n-2) u
+      for (int b = 0; b < 6; ++b) J[NJ - 1][b] -= f21 * pt.ms[i][b];
+      double as = fabs(ev.slip[i]);
+      double sgs = P.jac_exact ? 1.0 : sgn1(ev.slip[i]);
+      etau += (P.voche_m * (-1.0 / P.tau_v) * as / fabs(h_term) - as * n_ / tt * sgn1(h_term) * sgs) *
+              absh_m;
+    }
+  }
+  if (NJ == 7) {
+    double fac = P.theta_0 * pt.dg * n_ / tt;
+    for (int b = 0; b < 6; ++b) J[NJ - 1][b] *= fac;
+    J[NJ - 1][NJ - 1] = 1.0 - P.theta_0 * etau;
+  }
+  // IW(wp) (mm10_iw, mm10_b.f:1605-1634) and identity
+  J[0][3] += 2.0 * wp[2];
+  J[0][5] -= 2.0 * wp[1];
+  J[1][3] += 2.0 * wp[2];
+  J[1][4] -= 2.0 * wp[0];
+  J[2][4] += 2.0 * wp[0];
+  J[2][5] += 2.0 * wp[1];
+  J[3][0] += wp[2];
+  J[3][1] -= wp[2];
+  J[3][4] -= wp[1];
+  J[3][5] += wp[0];
+  J[4][1] += wp[0];
+  J[4][2] -= wp[0];
+  J[4][3] += wp[1];
+  J[4][5] += wp[2];
+  J[5][0] += wp[1];
+  J[5][2] -= wp[1];
+  J[5][3] += wp[0];
+  J[5][4] -= wp[2];
+  for (int a = 0; a < 6; ++a) J[a][a] += 1.0;
+}
+
+// mm10_setup (mm10_a.f:830-962) for a point with strain increment D, time tinc
+template <int NS>
+CPFFT_HD inline void setup_point(const MatParams& P, const Orient<NS>& o, const double R[3][3],
+                                 const StateN<NS>& n, const double D[6], double tinc,
+                                 Point<NS>& pt) {
+  for (int k = 0; k < 6; ++k) pt.D[k] = D[k];
+  pt.tinc = tinc;
+  double t1 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+  double t2 = D[3] * D[3] + D[4] * D[4] + D[5] * D[5];
+  pt.dg = P.alter_mode ? P.eps_dot_0_y * tinc : sqrt((2.0 / 3.0) * (t1 + 0.5 * t2));
+  double tn[3][3], RE[6][6], RWC[3][3], w[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) tn[i][j] = n.Rp[j][i];
+  rt2rve(tn, RE);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) w[i][j] = R[i][0] * tn[0][j] + R[i][1] * tn[1][j] + R[i][2] * tn[2][j];
+  rt2rvw(w, RWC);
+  for (int s = 0; s < NS; ++s) {
+    for (int a = 0; a < 6; ++a) {
+      double v = 0.0;
+      for (int b = 0; b < 6; ++b) v += RE[a][b] * o.ms[s][b];
+      pt.ms[s][a] = v;
+    }
+    for (int a = 0; a < 3; ++a)
+      pt.qc[s][a] = RWC[a][0] * o.qs[s][0] + RWC[a][1] * o.qs[s][1] + RWC[a][2] * o.qs[s][2];
+  }
+}
+
+// mm10_solve (mm10_a.f:2860-3295). x in/out; xprev receives the iterate at
+// which the last full Jacobian was formed. Returns fail.
+template <int NS>
+CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C,
+                                const Point<NS>& pt, const StateN<NS>& n, double dtinc,
+                                double x[7], double xprev[7], double* tt_rate, Counters& cnt) {
+  const double cls = 1.0e-4, red = 0.5;
+  const int mls = 10;
+  int fail = 0;
+  double inR1;
+  // ---- predictor: 6x6 Newton on R1 at extrapolated tt (mm10_a.f:2975-3156)
+  {
+    double d1[6], d2[6];
+    for (int k = 0; k < 6; ++k) { d1[k] = pt.D[k]; d2[k] = n.D[k]; }
+    double dtr = (d1[0] + d1[1] + d1[2]) / 3.0;
+    d1[0] -= dtr; d1[1] -= dtr; d1[2] -= dtr;
+    double t1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+    double t2 = d1[3] * d1[3] + d1[4] * d1[4] + d1[5] * d1[5];
+    double nn = (t1 + t2 == 0.0) ? 0.0 : 1.0 / sqrt(t1 + t2);
+    for (int k = 0; k < 6; ++k) d1[k] = (t1 + t2 == 0.0) ? 0.0 : d1[k] * nn;
+    dtr = (d2[0] + d2[1] + d2[2]) / 3.0;
+    d2[0] -= dtr; d2[1] -= dtr; d2[2] -= dtr;
+    t1 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+    t2 = d2[3] * d2[3] + d2[4] * d2[4] + d2[5] * d2[5];
+    nn = (t1 + t2 == 0.0) ? 0.0 : 1.0 / sqrt(t1 + t2);
+    for (int k = 0; k < 6; ++k) d2[k] = (t1 + t2 == 0.0) ? 0.0 : d2[k] * nn;
+    double cosang = 0.0;
+    for (int k = 0; k < 6; ++k) cosang += d1[k] * d2[k];
+    cosang = fmax(cosang, 0.0);
+    double x2 = x[6] + cosang * n.tt_rate * dtinc;
+    double x1[6], R1[6], wp[3];
+    for (int k = 0; k < 6; ++k) x1[k] = x[k];
+    SlipEval<NS> ev;
+    eval_slip(P, pt, x1, x2, ev);
+    form_R1(C, pt, n, x1, ev, R1, wp);
+    double dot = 0.0;
+    for (int k = 0; k < 6; ++k) dot += R1[k] * R1[k];
+    double nR1 = sqrt(dot);
+    inR1 = nR1;
+    int it = 0, pfail = 0;
+    while (nR1 > P.atol1 && nR1 / inR1 > P.rtol1) {
+      double J[6][6], A[6][6], dx[6][1];
+      form_J<NS, 6>(P, C, pt, x1, x2, J);
+      for (int a = 0; a < 6; ++a) {
+        dx[a][0] = R1[a];
+        for (int b = 0; b < 6; ++b) A[a][b] = -J[a][b];
+      }
+      lu_solve<6, 1>(A, dx);
+      double ls1 = 0.5 * dot, ls2 = 0.0;
+      for (int b = 0; b < 6; ++b) {
+        double wv = 0.0;
+        for (int a = 0; a < 6; ++a) wv += J[a][b] * R1[a];
+        ls2 += dx[b][0] * wv;
+      }
+      ls2 *= cls;
+      double alpha = 1.0;
+      for (int ls = 0;; ++ls) {
+        double nlsx = ls1 + ls2 * alpha, xn[6];
+        for (int k = 0; k < 6; ++k) xn[k] = x1[k] + alpha * dx[k][0];
+        eval_slip(P, pt, xn, x2, ev);
+        form_R1(C, pt, n, xn, ev, R1, wp);
+        cnt.ls_evals++;
+        dot = 0.0;
+        for (int k = 0; k < 6; ++k) dot += R1[k] * R1[k];
+        nR1 = sqrt(dot);
+        if (0.5 * dot <= nlsx || ls > mls) {
+          for (int k = 0; k < 6; ++k) x1[k] = xn[k];
+          break;
+        }
+        alpha *= red;
+      }
+      ++it;
+      bool nan = false;
+      for (int k = 0; k < 6; ++k) nan |= (x1[k] != x1[k]);
+      if (it > P.miter || nan) { pfail = 1; break; }
+    }
+    cnt.pred_it += it;
+    if (pfail) {
+      fail = 1;
+    } else {
+      for (int k = 0; k < 6; ++k) x[k] = x1[k];
+      x[6] = x2;
+    }
+  }
+  // ---- full Newton on [R1; R2] (mm10_a.f:3162-3293)
+  {
+    double R[7];
+    double dot = form_R<NS>(P, C, pt, n, x, R, tt_rate);
+    double nR = sqrt(dot), inR = nR;
+    if (inR == 0.0) inR = inR1;
+    int it = 0;
+    for (int k = 0; k < 7; ++k) xprev[k] = x[k];
+    while ((nR > P.atol && nR / inR > P.rtol) || it < 1) {
+      double J[7][7], A[7][7], dx[7][1];
+      form_J<NS, 7>(P, C, pt, x, x[6], J);
+      for (int k = 0; k < 7; ++k) xprev[k] = x[k];
+      for (int a = 0; a < 7; ++a) {
+        dx[a][0] = R[a];
+        for (int b = 0; b < 7; ++b) A[a][b] = -J[a][b];
+      }
+      lu_solve<7, 1>(A, dx);
+      double ls1 = 0.5 * dot, ls2 = 0.0;
+      for (int b = 0; b < 7; ++b) {
+        double wv = 0.0;
+        for (int a = 0; a < 7; ++a) wv += J[a][b] * R[a];
+        ls2 += dx[b][0] * wv;
+      }
+      ls2 *= cls;
+      double alpha = 1.0;
+      for (int ls = 0;; ++ls) {
+        double nlsx = ls1 + ls2 * alpha, xn[7];
+        for (int k = 0; k < 7; ++k) xn[k] = x[k] + alpha * dx[k][0];
+        dot = form_R<NS>(P, C, pt, n, xn, R, tt_rate);
+        cnt.ls_evals++;
+        nR = sqrt(dot);
+        if (0.5 * dot <= nlsx || ls > mls) {
+          for (int k = 0; k < 7; ++k) x[k] = xn[k];
+          break;
+        }
+        alpha *= red;
+      }
+      ++it;
+      bool nan = false;
+      for (int k = 0; k < 7; ++k) nan |= (x[k] != x[k]);
+      if (it > P.miter || nan) { fail = 1; break; }
+    }
+    cnt.upd_it += it;
+  }
+  return fail;
+}
+
+// ------------------------------------------------------- exact dP/dF
+// cep2A_a (cep2A.f:86-284) for one voxel; C = consistent tangent (row-major
+// 6x6), t = unrotated Cauchy stress (Voigt), K written column (k,l) at a time.
+template <class Store>
+CPFFT_HD inline void dPdF(const double Fn[3][3], const double Fn1[3][3], const double t6[6],
+                          const double C[6][6], const double Rh[3][3], const double fnhinv[3][3],
+                          const double R[3][3], const double finv[3][3], double J,
+                          Store&& store) {
+  double t[3][3] = {{t6[0], t6[3], t6[5]}, {t6[3], t6[1], t6[4]}, {t6[5], t6[4], t6[2]}};
+  auto mm = [](const double A[3][3], const double B[3][3], double Cc[3][3]) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Cc[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+  };
+  auto det = [](const double A[3][3]) {
+    return A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
+           A[0][0] * A[1][2] * A[2][1] - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+  };
+  double Rt[3][3], U[3][3], Y[3][3], tmp[3][3], sigma[3][3], Uinv[3][3], tUi[3][3], RT_[3][3];
+  double RY[3][3], RYR[3][3], RtRF[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Rt[i][j] = R[j][i];
+  mm(Rt, Fn1, U);
+  double trU = U[0][0] + U[1][1] + U[2][2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Y[i][j] = (i == j ? trU : 0.0) - U[i][j];
+  mm(R, t, RT_);
+  mm(RT_, Rt, sigma);
+  mm(finv, R, Uinv);
+  mm(t, Uinv, tUi);
+  mm(R, Y, RY);
+  mm(RY, Rt, RYR);
+  double dY1 = 1.0 / det(Y);
+  mm(RT_, Uinv, RtRF);
+  double Rht[3][3], Fnh[3][3], dFn[3][3], Uh[3][3], Yh[3][3], RYRh[3][3], RYh[3][3], FpF[3][3];
+  double L[3][3], D2[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      Rht[i][j] = Rh[j][i];
+      Fnh[i][j] = 0.5 * (Fn1[i][j] + Fn[i][j]);
+      dFn[i][j] = Fn1[i][j] - Fn[i][j];
+      FpF[i][j] = 0.5 * fnhinv[i][j];
+    }
+  mm(Rht, Fnh, Uh);
+  double trUh = Uh[0][0] + Uh[1][1] + Uh[2][2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Yh[i][j] = (i == j ? trUh : 0.0) - Uh[i][j];
+  mm(Rh, Yh, RYh);
+  mm(RYh, Rht, RYRh);
+  double dYh1 = 0.5 / det(Yh);
+  mm(dFn, FpF, L);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D2[i][j] = L[i][j] + L[j][i];
+  const int vg[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) {
+      // dR(i,m)/dF(k,l), dRh(i,m)/dF(k,l), dL(i,j)/dF(k,l)
+      double dR[3][3], dRh[3][3], dL[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int m = 0; m < 3; ++m) {
+          dR[i][m] = dY1 * (RYR[i][k] * Y[l][m] - RY[i][l] * RY[k][m]);
+          dRh[i][m] = dYh1 * (RYRh[i][k] * Yh[l][m] - RYh[i][l] * RYh[k][m]);
+          dL[i][m] = ((i == k) ? FpF[l][m] : 0.0) - L[i][k] * FpF[l][m];
+        }
+      // dt(i,j)/dF(k,l): dRh^T D2 Rh + Rh^T (dL + dL^T) Rh + Rh^T D2 dRh
+      double S[3][3], A1[3][3], A2[3][3], dd[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S[i][j] = dL[i][j] + dL[j][i];
+      mm(D2, Rh, A1);   // D2 Rh
+      mm(S, Rh, A2);    // S Rh
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = 0.0;
+          for (int m = 0; m < 3; ++m) v += dRh[m][i] * A1[m][j] + Rh[m][i] * A2[m][j];
+          double w = 0.0;
+          for (int m = 0; m < 3; ++m) {
+            double dm = 0.0;
+            for (int q = 0; q < 3; ++q) dm += D2[m][q] * dRh[q][j];
+            w += Rh[m][i] * dm;
+          }
+          dd[i][j] = v + w;
+        }
+      // tangent contraction dt'(i,j) = sum_ab cep99(ij,ab) dd(b,a)
+      double dt[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = 0.0;
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) v += C[vg[a][b]][vg[i][j]] * dd[b][a];
+          dt[i][j] = v;
+        }
+      // dP(i,j)/dF(k,l)
+      double Rdt[3][3], RdtU[3][3];
+      mm(R, dt, Rdt);
+      mm(Rdt, Uinv, RdtU);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = RtRF[i][j] * (J * finv[l][k]) + J * RdtU[i][j];
+          for (int m = 0; m < 3; ++m) {
+            v -= J * sigma[i][m] * finv[j][k] * finv[l][m];
+            v += J * dR[i][m] * tUi[m][j];
+            double s4 = 0.0;
+            for (int q = 0; q < 3; ++q) s4 += dR[q][m] * finv[j][q];
+            v += J * RT_[i][m] * s4;
+          }
+          store(i * 27 + j * 9 + k * 3 + l, v);
+        }
+    }
+}
+
+// ------------------------------------------------------- voxel update
+struct VoxelIO {
+  double Fn[3][3], Fn1[3][3];
+};
+
+// Full per-voxel update.  sn: state n words (stride sstride), writes state
+// n+1 via snp1, P and K through the store functors.
+template <int NS, class LoadS, class StoreS, class StoreP, class StoreK>
+CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int step, int iter,
+                                  const VoxelIO& io, LoadS&& ldn, StoreS&& stn1, StoreP&& stP,
+                                  StoreK&& stK, Counters& cnt) {
+  // ---- kinematics (drive_eps_sig.f:207-245)
+  double fnh[3][3], dfn[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      fnh[i][j] = 0.5 * (io.Fn[i][j] + io.Fn1[i][j]);
+      dfn[i][j] = io.Fn1[i][j] - io.Fn[i][j];
+    }
+  double rnh[3][3], rn1[3][3], fnhinv[3][3];
+  polar_R(fnh, rnh);
+  polar_R(io.Fn1, rn1);
+  inv33(fnh, fnhinv);
+  double ddt[6], uddt[6];
+  {
+#pragma clang fp contract(off)
+    const double(&A)[3][3] = dfn;
+    const double(&B)[3][3] = fnhinv;
+    ddt[0] = A[0][0] * B[0][0] + A[0][1] * B[1][0] + A[0][2] * B[2][0];
+    ddt[3] = A[1][0] * B[0][0] + A[1][1] * B[1][0] + A[1][2] * B[2][0] + A[0][0] * B[0][1] +
+             A[0][1] * B[1][1] + A[0][2] * B[2][1];
+    ddt[5] = A[2][0] * B[0][0] + A[2][1] * B[1][0] + A[2][2] * B[2][0] + A[0][0] * B[0][2] +
+             A[0][1] * B[1][2] + A[0][2] * B[2][2];
+    ddt[1] = A[1][0] * B[0][1] + A[1][1] * B[1][1] + A[1][2] * B[2][1];
+    ddt[4] = A[2][0] * B[0][1] + A[2][1] * B[1][1] + A[2][2] * B[2][1] + A[1][0] * B[0][2] +
+             A[1][1] * B[1][2] + A[1][2] * B[2][2];
+    ddt[2] = A[2][0] * B[0][2] + A[2][1] * B[1][2] + A[2][2] * B[2][2];
+    // getrm1 opt 1 + qmply1: uddt = Q(rnh) ddt
+    const double(&r)[3][3] = rnh;
+    double q[6][6];
+    for (int a = 0; a < 3; ++a) {
+      q[a][0] = r[0][a] * r[0][a];
+      q[a][1] = r[1][a] * r[1][a];
+      q[a][2] = r[2][a] * r[2][a];
+      q[a][3] = r[0][a] * r[1][a];
+      q[a][4] = r[2][a] * r[1][a];
+      q[a][5] = r[0][a] * r[2][a];
+    }
+    const int ia[3] = {0, 1, 0}, ib[3] = {1, 2, 2};
+    for (int z = 0; z < 3; ++z) {
+      int a = ia[z], b = ib[z];
+      q[3 + z][0] = 2.0 * r[0][a] * r[0][b];
+      q[3 + z][1] = 2.0 * r[1][a] * r[1][b];
+      q[3 + z][2] = 2.0 * r[2][a] * r[2][b];
+      q[3 + z][3] = r[0][a] * r[1][b] + r[0][b] * r[1][a];
+      q[3 + z][4] = r[1][a] * r[2][b] + r[2][a] * r[1][b];
+      q[3 + z][5] = r[0][a] * r[2][b] + r[2][a] * r[0][b];
+    }
+    // reference index pattern for rows 4..6 (polar.f:735-758)
+    q[4][1] = 2.0 * r[1][2] * r[1][1];
+    q[4][2] = 2.0 * r[2][1] * r[2][2];
+    q[4][3] = r[0][1] * r[1][2] + r[1][1] * r[0][2];
+    q[4][5] = r[0][1] * r[2][2] + r[2][1] * r[0][2];
+    q[5][3] = r[0][0] * r[1][2] + r[1][0] * r[0][2];
+    q[3][4] = r[1][0] * r[2][1] + r[2][0] * r[1][1];
+    for (int j = 0; j < 6; ++j)
+      uddt[j] = q[j][0] * ddt[0] + q[j][1] * ddt[1] + q[j][2] * ddt[2] + q[j][3] * ddt[3] +
+                q[j][4] * ddt[4] + q[j][5] * ddt[5];
+  }
+
+  // ---- state n (mm10_copy_cc_hist; re-initialised at step 1, mm10_a.f:89-95)
+  StateN<NS> n;
+  double slipsum_n[NS], work_n[3];
+  if (step == 1) {
+    for (int k = 0; k < 6; ++k) { n.stress[k] = 0.0; n.D[k] = 0.0; }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) n.Rp[i][j] = (i == j) ? 1.0 : 0.0;
+    n.tt = P.tau_y + 1.0e-5;
+    n.tt_rate = ldn(16);
+    for (int i = 0; i < NS; ++i) slipsum_n[i] = 0.0;
+    work_n[0] = work_n[1] = work_n[2] = 0.0;
+  } else {
+    for (int k = 0; k < 6; ++k) n.stress[k] = ldn(k);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) n.Rp[i][j] = ldn(6 + 3 * j + i);
+    n.tt = ldn(15);
+    n.tt_rate = ldn(16);
+    for (int k = 0; k < 6; ++k) n.D[k] = ldn(17 + k);
+    for (int i = 0; i < NS; ++i) slipsum_n[i] = ldn(23 + NS + i);
+    for (int k = 0; k < 3; ++k) work_n[k] = ldn(23 + 2 * NS + k);
+  }
+
+  const double* __restrict__ C = o.C;
+  Point<NS> pt;
+  setup_point(P, o, rn1, n, uddt, P.dt, pt);
+
+  double s1[6], tt1, ttrate1 = 0.0;
+  double tangent[6][6];
+  double Rp1[3][3];
+  double slip1[NS];
+  double winc = 0.0, pwinc = 0.0, psinc = 0.0;
+  int failed = 0;
+  for (int i = 0; i < NS; ++i) slip1[i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Rp1[i][j] = 0.0;
+
+  double temp1 = 0.0, temp2 = 0.0;
+  for (int k = 0; k < 6; ++k) { temp1 += uddt[k] * uddt[k]; temp2 += n.stress[k] * n.stress[k]; }
+  const bool no_load = (temp2 == 0.0) && (temp1 == 0.0);
+  if (iter == 0 || no_load) {
+    // explicit update with the elastic tangent (mm10_a.f:2735-2753)
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) tangent[a][b] = C[a * 6 + b];
+    for (int k = 0; k < 6; ++k) s1[k] = n.stress[k];
+    if (!no_load) {
+      SlipEval<NS> ev;
+      double R1[6], wp[3];
+      eval_slip(P, pt, n.stress, n.tt, ev);
+      form_R1(C, pt, n, n.stress, ev, R1, wp);
+      for (int k = 0; k < 6; ++k) s1[k] = n.stress[k] - R1[k];
+    }
+    tt1 = n.tt;
+  } else {
+    // adaptive sub-stepping (mm10_a.f:2759-2843)
+    double x[7], xprev[7], ost[6], ott = n.tt;
+    for (int k = 0; k < 6; ++k) { x[k] = n.stress[k]; ost[k] = x[k]; }
+    double frac = 0.0, stp = 1.0;
+    int cuts = 0, fail = 0;
+    Point<NS> cur;
+    while (frac < 1.0) {
+      double Dw[6];
+      for (int k = 0; k < 6; ++k) Dw[k] = uddt[k] * (stp + frac);
+      setup_point(P, o, rn1, n, Dw, P.dt * (stp + frac), cur);
+      x[6] = n.tt;
+      fail = local_solve<NS>(P, C, cur, n, P.dt * stp, x, xprev, &ttrate1, cnt);
+      if (fail) {
+        for (int k = 0; k < 6; ++k) x[k] = ost[k];
+        x[6] = ott;
+        stp *= 0.5;
+        ++cuts;
+        cnt.cuts++;
+        if (cuts > 4) break;
+        fail = 0;
+      } else {
+        for (int k = 0; k < 6; ++k) ost[k] = x[k];
+        ott = x[6];
+        frac += stp;
+      }
+    }
+    bool bad = fail || (x[6] != x[6]);
+    for (int k = 0; k < 6; ++k) bad |= (x[k] != x[k]);
+    if (bad) {
+      // The reference abandons the rest of its element block here
+      // (mm10_a.f:113-116, 255-258); this implementation reverts the voxel
+      // to state n with the elastic tangent and counts it.
+      failed = 1;
+      for (int k = 0; k < 6; ++k) s1[k] = n.stress[k];
+      tt1 = n.tt;
+      ttrate1 = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Rp1[i][j] = n.Rp[i][j];
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) tangent[a][b] = C[a * 6 + b];
+    } else {
+      for (int k = 0; k < 6; ++k) s1[k] = x[k];
+      tt1 = x[6];
+      // consistent tangent (mm10_tangent) from the Jacobian at xprev for the
+      // last sub-step point, symmetrised (mm10_a.f:1134-1142)
+      {
+        double J[7][7];
+        form_J<NS, 7>(P, C, cur, xprev, xprev[6], J);
+        double JJ[6][6], B[6][6];
+        double j22 = J[6][6];
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) JJ[a][b] = J[a][b] - J[a][6] * (J[6][b] / j22);
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) B[a][b] = C[a * 6 + b];
+        lu_solve<6, 6>(JJ, B);
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) tangent[a][b] = 0.5 * (B[a][b] + B[b][a]);
+      }
+      // plastic rotation (mm10_update_rotation): Rp = expm(W(wbarp)) Rp_n
+      SlipEval<NS> ev;
+      eval_slip(P, pt, s1, tt1, ev);
+      {
+        double RW[3][3], tn[3][3];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) tn[i][j] = n.Rp[j][i];
+        rt2rvw(tn, RW);
+        double wb[3] = {0, 0, 0};
+        for (int s = 0; s < NS; ++s) {
+          double qs[3];
+          for (int a = 0; a < 3; ++a)
+            qs[a] = RW[a][0] * o.qs[s][0] + RW[a][1] * o.qs[s][1] + RW[a][2] * o.qs[s][2];
+          for (int a = 0; a < 3; ++a) wb[a] += ev.slip[s] * qs[a];
+        }
+        double W[3][3] = {{0.0, wb[2], wb[1]}, {-wb[2], 0.0, wb[0]}, {-wb[1], -wb[0], 0.0}};
+        double al = sqrt(W[1][2] * W[1][2] + W[0][2] * W[0][2] + W[0][1] * W[0][1]);
+        double E[3][3];
+        if (al < 1.0e-16) {
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) E[i][j] = 0.0;
+        } else {
+          double ca = (1.0 - cos(al)) / (al * al), sa = sin(al) / al;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              E[i][j] = sa * W[i][j] + ca * (W[i][0] * W[0][j] + W[i][1] * W[1][j] + W[i][2] * W[2][j]);
+        }
+        E[0][0] += 1.0; E[1][1] += 1.0; E[2][2] += 1.0;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            Rp1[i][j] = E[i][0] * n.Rp[0][j] + E[i][1] * n.Rp[1][j] + E[i][2] * n.Rp[2][j];
+      }
+      // outputs kept in history (mm10_output): slip increments, work terms
+      for (int s = 0; s < NS; ++s) slip1[s] = ev.slip[s];
+      winc = s1[0] * uddt[0] + s1[1] * uddt[1] + s1[2] * uddt[2] + s1[3] * uddt[3] +
+             s1[4] * uddt[4] + s1[5] * uddt[5];
+      double ee[6];
+      {
+        double S[6][6], b[6][1];
+        for (int a = 0; a < 6; ++a) {
+          b[a][0] = s1[a];
+          for (int c = 0; c < 6; ++c) S[a][c] = C[a * 6 + c];
+        }
+        lu_solve<6, 1>(S, b);  // C^-1 sigma (SPD solve, mm10_a.f:3537)
+        double er[6][6];
+        rt2rve(rn1, er);
+        for (int a = 0; a < 6; ++a) {
+          double v = 0.0;
+          for (int c = 0; c < 6; ++c) v += er[a][c] * b[c][0];
+          ee[a] = v;
+        }
+      }
+      double dbar[6] = {0, 0, 0, 0, 0, 0}, wp[3] = {0, 0, 0}, ewwe[6], ep[6];
+      for (int s = 0; s < NS; ++s) {
+        for (int k = 0; k < 6; ++k) dbar[k] += slip1[s] * pt.ms[s][k];
+        for (int k = 0; k < 3; ++k) wp[k] += slip1[s] * pt.qc[s][k];
+      }
+      symsw(ee, wp, ewwe);
+      for (int k = 0; k < 6; ++k) ep[k] = dbar[k] + ewwe[k];
+      double a1 = ep[0] * ep[0] + ep[1] * ep[1] + ep[2] * ep[2];
+      double a2 = ep[3] * ep[3] + ep[4] * ep[4] + ep[5] * ep[5];
+      psinc = sqrt(2.0 / 3.0 * (a1 + 0.5 * a2));
+      pwinc = s1[0] * ep[0] + s1[1] * ep[1] + s1[2] * ep[2] + s1[3] * ep[3] + s1[4] * ep[4] +
+              s1[5] * ep[5];
+    }
+  }
+  cnt.failed += failed;
+
+  // ---- store state n+1 (mm10_store_cryhist, mm10_a_store_crystal)
+  for (int k = 0; k < 6; ++k) stn1(k, s1[k]);
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) stn1(6 + 3 * j + i, Rp1[i][j]);
+  stn1(15, tt1);
+  stn1(16, ttrate1);
+  for (int k = 0; k < 6; ++k) stn1(17 + k, uddt[k]);
+  for (int s = 0; s < NS; ++s) stn1(23 + s, slip1[s]);
+  for (int s = 0; s < NS; ++s) stn1(23 + NS + s, slipsum_n[s] + slip1[s]);
+  stn1(23 + 2 * NS + 0, work_n[0] + winc);
+  stn1(23 + 2 * NS + 1, work_n[1] + pwinc);
+  stn1(23 + 2 * NS + 2, work_n[2] + psinc);
+
+  // ---- Cauchy stress and P = J sigma F^-T (drive_eps_sig.f:262-273)
+  double fn1inv[3][3];
+  double detF = inv33(io.Fn1, fn1inv);
+  double cs[3][3];
+  {
+    double ts[3][3] = {{s1[0], s1[3], s1[5]}, {s1[3], s1[1], s1[4]}, {s1[5], s1[4], s1[2]}};
+    double tmp[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        tmp[i][j] = rn1[i][0] * ts[0][j] + rn1[i][1] * ts[1][j] + rn1[i][2] * ts[2][j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        cs[i][j] = tmp[i][0] * rn1[j][0] + tmp[i][1] * rn1[j][1] + tmp[i][2] * rn1[j][2];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      stP(3 * i + j, detF * (cs[i][0] * fn1inv[j][0] + cs[i][1] * fn1inv[j][1] + cs[i][2] * fn1inv[j][2]));
+
+  // ---- K4 = dP/dF (cep2A)
+  dPdF(io.Fn, io.Fn1, s1, tangent, rnh, fnhinv, rn1, fn1inv, detF, stK);
+}
+
+}  // namespace cpfft

@@ -1,0 +1,284 @@
+! fft_nr3.f90 -- the Fortran driver of the load-step loop, running on the
+! MI355X operators of libcpfft_amd.so.
+!
+! It keeps the control flow of the reference's `compute` command
+! (FFT_finite_3d.f:138-147: drive_eps_sig(1,0) then FFT_nr3) and of
+! FFT_nr3 (FFT_nr3.f:14-200): mixed boundary conditions through
+! NBC_update, the initial-residual solve with the old tangent, the Newton
+! loop on the compatible correction dFm with at least one iteration, the
+! post-convergence stress update, the P_bar check, tangent_homo and the
+! state roll-over.  Fields never leave the device; the host keeps only the
+! 9-vectors and the 9x9 system of NBC_update.
+module cpfft_driver
+  use iso_c_binding
+  use cpfft_bindings
+  implicit none
+  private
+  public :: cpfft_compute
+
+contains
+
+  ! NBC_update (FFT_nr3.f:375-433): rows of prescribed P take the
+  ! homogenised tangent, rows of prescribed F the identity; solved by
+  ! Gaussian elimination with partial pivoting (dgesv).
+  subroutine NBC_update(C_homo, DbarF, P_bar, PBC, isNBC, info)
+    real(c_double), intent(in)    :: C_homo(9,9), P_bar(9), PBC(9)
+    logical, intent(in)           :: isNBC(9)
+    real(c_double), intent(inout) :: DbarF(9)
+    integer, intent(out)          :: info
+    real(c_double) :: AAA(9,9), bbb(9), t, l
+    integer :: i, j, k, p
+
+    do i = 1, 9
+      if (isNBC(i)) then
+        AAA(i,1:9) = C_homo(1:9,i)
+        bbb(i) = PBC(i) - P_bar(i)
+      else
+        AAA(i,1:9) = 0.0d0
+        AAA(i,i) = 1.0d0
+        bbb(i) = DbarF(i)
+      end if
+    end do
+    info = 0
+    do j = 1, 9
+      p = j
+      do i = j + 1, 9
+        if (abs(AAA(i,j)) > abs(AAA(p,j))) p = i
+      end do
+      if (AAA(p,j) == 0.0d0) then
+        info = j
+        return
+      end if
+      if (p /= j) then
+        do k = 1, 9
+          t = AAA(j,k); AAA(j,k) = AAA(p,k); AAA(p,k) = t
+        end do
+        t = bbb(j); bbb(j) = bbb(p); bbb(p) = t
+      end if
+      do i = j + 1, 9
+        l = AAA(i,j) / AAA(j,j)
+        do k = j + 1, 9
+          AAA(i,k) = AAA(i,k) - l * AAA(j,k)
+        end do
+        bbb(i) = bbb(i) - l * bbb(j)
+      end do
+    end do
+    do i = 9, 1, -1
+      t = bbb(i)
+      do k = i + 1, 9
+        t = t - AAA(i,k) * bbb(k)
+      end do
+      bbb(i) = t / AAA(i,i)
+    end do
+    DbarF(1:9) = bbb(1:9)
+  end subroutine NBC_update
+
+  real(c_double) function wall()
+    integer(8) :: cnt, rate
+    call system_clock(cnt, rate)
+    wall = dble(cnt) / dble(rate)
+  end function wall
+
+  ! compute + FFT_nr3.  BC_all(9,nstep) as filled by inlod (inlod.f:57-63);
+  ! returns 0 or the negative code of the reference's die_abort paths:
+  ! -2 CG not converged (FFT_nr3.f:335-339), -3 NBC_update singular
+  ! (:418-421), -4 Newton maxIter (:116-119), -5 mixed-BC maxIter
+  ! (:153-156), other negatives from the operators.
+  ! counters(:,step) = Newton iterations, CG iterations, mixed-BC loops,
+  ! drive_eps_sig calls; times = pcg, sig-eps, total wall seconds (thyme
+  ! categories 1 and 2, thyme.f:15-47).
+  integer(c_int) function cpfft_compute(ctx, nstep, BC_all, isNBC_c, tolPCG, tolNR, maxIter, &
+       verbose, Pbar_out, counters, times) bind(C, name="cpfft_compute")
+    type(c_ptr), value     :: ctx
+    integer(c_int), value  :: nstep, maxIter, verbose
+    real(c_double), value  :: tolPCG, tolNR
+    real(c_double)         :: BC_all(9, nstep), Pbar_out(9, nstep), times(3)
+    integer(c_int)         :: isNBC_c(9), counters(4, nstep)
+
+    type(c_ptr) :: Fn1, Pn1, dFm, b
+    real(c_double) :: P_bar(9), resPbar(3), DbarF(9), barF(9), barF_t(9)
+    real(c_double) :: FBC(9), PBC(9), C_homo(81), Fnorm, resfft, relres, dn, t0, t1
+    integer(c_int) :: rc, its
+    integer(c_long_long) :: cnt0(8), cnt1(8)
+    integer :: step, ii, iiter_EBC, iiter_NBC, info, nnewton, ncalls, nloops
+    logical :: isNBC(9), existNBC
+
+    cpfft_compute = 0
+    times = 0.0d0
+    counters = 0
+    Pbar_out = 0.0d0
+    isNBC = isNBC_c /= 0
+    existNBC = any(isNBC)
+    barF_t = [1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0]
+    P_bar = 0.0d0
+    Fn1 = cpfft_field(ctx, CPFFT_FN1)
+    Pn1 = cpfft_field(ctx, CPFFT_PN1)
+    dFm = cpfft_field(ctx, CPFFT_DFM)
+    b = cpfft_field(ctx, CPFFT_B)
+    times(3) = wall()
+
+    ! compute (FFT_finite_3d.f:145): elastic tangent at F = I
+    t0 = wall()
+    rc = cpfft_drive_eps_sig(ctx, 1, 0)
+    if (rc < 0) then
+      cpfft_compute = rc
+      return
+    end if
+    times(2) = times(2) + (wall() - t0)
+
+    ! initial homogenised tangent (FFT_nr3.f:47)
+    t0 = wall()
+    rc = cpfft_tangent_homo(ctx, tolPCG, C_homo)
+    times(1) = times(1) + (wall() - t0)
+    if (rc < 0) then
+      cpfft_compute = rc
+      return
+    end if
+
+    do step = 1, nstep
+      if (verbose /= 0) write(*, 1000) step
+      rc = cpfft_counters(ctx, cnt0)
+      nnewton = 0
+      ncalls = 0
+      nloops = 0
+      PBC = 0.0d0
+      FBC = 0.0d0
+      DbarF = 0.0d0
+      do ii = 1, 9
+        if (isNBC(ii)) then
+          PBC(ii) = BC_all(ii, step)
+        else
+          FBC(ii) = BC_all(ii, step)
+          DbarF(ii) = FBC(ii) - barF_t(ii)
+        end if
+      end do
+      if (existNBC) then
+        call NBC_update(C_homo, DbarF, P_bar, PBC, isNBC, info)
+        if (info /= 0) then
+          cpfft_compute = -3
+          return
+        end if
+      end if
+      barF = barF_t + DbarF
+      iiter_NBC = 0
+
+      do while (.true.)
+        ! initial residual: distribute DbarF over the grid (FFT_nr3.f:85-96)
+        rc = cpfft_fill9(ctx, dFm, DbarF)
+        rc = cpfft_dnrm2(ctx, Fn1, Fnorm)
+        rc = cpfft_daxpy(ctx, 1.0d0, dFm, Fn1)
+        t0 = wall()
+        rc = cpfft_gkdf(ctx, dFm, b, 1)
+        rc = cpfft_dscal(ctx, -1.0d0, b)
+        rc = cpfft_pcg(ctx, b, dFm, tolPCG, its, relres)
+        times(1) = times(1) + (wall() - t0)
+        if (rc < -1) then
+          cpfft_compute = -2
+          return
+        end if
+        rc = cpfft_daxpy(ctx, 1.0d0, dFm, Fn1)
+        rc = cpfft_dnrm2(ctx, dFm, dn)
+        resfft = dn / Fnorm
+        if (verbose /= 0) write(*, 1003) resfft
+        iiter_EBC = 0
+        resfft = 1.0d0          ! at least one iteration (FFT_nr3.f:103)
+
+        do while (resfft > tolNR)
+          t0 = wall()
+          rc = cpfft_drive_eps_sig(ctx, step, iiter_EBC)
+          t1 = wall()
+          times(2) = times(2) + (t1 - t0)
+          ncalls = ncalls + 1
+          if (rc < 0) then
+            cpfft_compute = rc
+            return
+          end if
+          rc = cpfft_gkdf(ctx, Pn1, b, 0)
+          rc = cpfft_dscal(ctx, -1.0d0, b)
+          rc = cpfft_pcg(ctx, b, dFm, tolPCG, its, relres)
+          times(1) = times(1) + (wall() - t1)
+          if (rc < -1) then
+            cpfft_compute = -2
+            return
+          end if
+          rc = cpfft_daxpy(ctx, 1.0d0, dFm, Fn1)
+          rc = cpfft_dnrm2(ctx, dFm, dn)
+          resfft = dn / Fnorm
+          if (verbose /= 0) write(*, 1001) iiter_EBC, resfft
+          if (iiter_EBC == maxIter) then
+            cpfft_compute = -4
+            return
+          end if
+          iiter_EBC = iiter_EBC + 1
+          nnewton = nnewton + 1
+        end do
+
+        ! post-convergence update (FFT_nr3.f:123)
+        t0 = wall()
+        rc = cpfft_drive_eps_sig(ctx, step, iiter_EBC)
+        times(2) = times(2) + (wall() - t0)
+        ncalls = ncalls + 1
+        if (rc < 0) then
+          cpfft_compute = rc
+          return
+        end if
+
+        ! P_bar check (FFT_nr3.f:127-149)
+        rc = cpfft_mean9(ctx, Pn1, P_bar)
+        resPbar = 0.0d0
+        do ii = 1, 9
+          resPbar(2) = resPbar(2) + P_bar(ii) * P_bar(ii)
+          if (.not. isNBC(ii)) cycle
+          resPbar(1) = resPbar(1) + (P_bar(ii) - PBC(ii)) * (P_bar(ii) - PBC(ii))
+        end do
+        if (resPbar(2) < 1.0d-8) then
+          resPbar(3) = sqrt(resPbar(1))
+        else
+          resPbar(3) = sqrt(resPbar(1) / resPbar(2))
+        end if
+        nloops = nloops + 1
+        if (verbose /= 0) write(*, 1002) iiter_NBC, resPbar(3)
+        if (resPbar(3) <= tolNR) exit
+        if (iiter_NBC > maxIter) then
+          cpfft_compute = -5
+          return
+        end if
+        ! new homogenised tangent and macroscopic correction (FFT_nr3.f:162-167)
+        t0 = wall()
+        rc = cpfft_tangent_homo(ctx, tolPCG, C_homo)
+        times(1) = times(1) + (wall() - t0)
+        if (rc < 0) then
+          cpfft_compute = rc
+          return
+        end if
+        DbarF = 0.0d0
+        call NBC_update(C_homo, DbarF, P_bar, PBC, isNBC, info)
+        if (info /= 0) then
+          cpfft_compute = -3
+          return
+        end if
+        barF = barF + DbarF
+        iiter_NBC = iiter_NBC + 1
+      end do
+
+      ! roll-over (FFT_nr3.f:173-179)
+      barF_t(1:9) = barF(1:9)
+      rc = cpfft_update(ctx)
+      Pbar_out(:, step) = P_bar
+      rc = cpfft_counters(ctx, cnt1)
+      counters(1, step) = nnewton
+      counters(2, step) = int(cnt1(2) - cnt0(2))
+      counters(3, step) = nloops
+      counters(4, step) = ncalls
+    end do
+    rc = cpfft_sync(ctx)
+    times(3) = wall() - times(3)
+
+1000 format(/,4x,'--------------------------------------------------', &
+         '-----------------------',/,5x,'Now starting step: ',i7)
+1001 format(7x,'Iteration ', i5, 7x, '  residual ', d10.3)
+1002 format(7x,'Stress iteration ',i5, '  residual ',d10.3)
+1003 format(7x,'Initial residual ', 16x, d10.3)
+  end function cpfft_compute
+
+end module cpfft_driver

@@ -1,0 +1,153 @@
+/*
+ * cpfft.h -- C-ABI of the MI355X implementation of CPFFT's finite-strain
+ * FFT solver step (libcpfft_amd.so).
+ *
+ * The reference is a Fortran program whose hot path is the load-step loop
+ * FFT_nr3 (src/FFT_nr3.f:14-200) around three operators:
+ *   G_K_dF         src/G_K_dF.f:11-87        projection Gamma:(K4:F)
+ *   fftPcg         src/FFT_nr3.f:214-360     conjugate gradients on it
+ *   drive_eps_sig  src/drive_eps_sig.f:16-45 per-voxel stress update
+ * plus tangent_homo (src/tangent_homo.f:11-73), update (src/update.f:16)
+ * and the BLAS-1 calls FFT_nr3 makes (dnrm2/daxpy/dscal/dcopy).
+ *
+ * Each entry point below replaces one of those routines.  All state lives
+ * in device memory owned by an opaque context; fields are exchanged as
+ * device pointers (plain addresses, obtained from cpfft_field()).  The
+ * reference's module globals (module fft, src/mod_fft.f:1-79) become
+ * context fields with the same names.  A Fortran caller binds these with
+ * ISO_C_BINDING interfaces (see cpfft_amd/fortran/cpfft_bindings.f90).
+ *
+ * Layout (identical to the reference's Fortran arrays F(N3,9), K4(N3,81)):
+ *   second-order field  [9][V]  component c = 3(i-1)+j  (11,12,13,21,...)
+ *   K4                  [81][V] index 27(i-1)+9(j-1)+3(k-1)+l
+ *   voxel               e = ix*N*N + iy*N + iz (x slowest)
+ *   material state      [W][V]  W = 26 + 2*nslip words, see CPFFT_W_*
+ *
+ * Errors: every call returns 0 on success, a positive number for a
+ * non-fatal condition (CG not converged, local update failures) and a
+ * negative number for a fatal error; cpfft_last_error() describes it.
+ */
+#ifndef CPFFT_H
+#define CPFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cpfft_ctx cpfft_ctx;
+
+/* Crystal plasticity material (one crystal per voxel, Voce hardening).
+ * Mirrors the crystal properties the deck command `crystal` sets
+ * (src/incrystal.f, defaults src/mod_crystals.f:231-410). */
+typedef struct {
+  int32_t slip_set;   /* 1 = fcc {111}<110> (12), 8 = bcc {110}+{112} <111> (24) */
+  double e, nu, mu;   /* cubic elasticity through the compliance (mod_crystals.f:1833-1879) */
+  double rate_n;      /* harden_n: power-law rate exponent                   */
+  double tau_y, tau_v, theta_0, voche_m;   /* Voce hardening                 */
+  double gamma_bar;   /* eps_dot_0_y, reference slip rate (alter_mode)        */
+  int32_t alter_mode; /* 1: dg = gamma_bar * dt (mm10_a.f:2073)               */
+  double atol, atol1, rtol, rtol1;         /* local Newton tolerances          */
+  int32_t miter;      /* local Newton iteration limit                         */
+  int32_t jac_exact;  /* 0: local Jacobian J22 exactly as mm10_ehard_voche
+                         (mm10_b.f:1951-1983) writes it; 1: its exact
+                         derivative (no spurious sign(slipinc) factor)        */
+} cpfft_material;
+
+/* Green operator variants (G_K_dF.f:11-87 with FFT_init.f:272-385) */
+enum {
+  CPFFT_GAMMA_REFERENCE = 0, /* as the reference: for even N the phase shift of
+                                Nhalf = N/2+1 bins pairs frequency f with Gamma(f-1) */
+  CPFFT_GAMMA_TEXTBOOK = 1   /* orthogonal projection: Gamma(f), Nyquist planes
+                                removed; identical to the reference for odd N        */
+};
+
+/* device fields owned by the context (module fft globals, mod_fft.f:51-53) */
+enum {
+  CPFFT_FN = 0, CPFFT_FN1 = 1, CPFFT_PN = 2, CPFFT_PN1 = 3, CPFFT_DFM = 4, CPFFT_B = 5,
+  CPFFT_K4 = 6, CPFFT_STATE_N = 7, CPFFT_STATE_NP1 = 8, CPFFT_WORK = 9,
+  CPFFT_NFIELDS = 10
+};
+
+/* material state words (history words of mm10, mm10_d.f:174-254) */
+#define CPFFT_W_SIG 0            /* unrotated Cauchy stress, Voigt 6           */
+#define CPFFT_W_RP 6             /* plastic rotation, column-major 9           */
+#define CPFFT_W_TT 15            /* slip resistance tau_tilde                  */
+#define CPFFT_W_TTRATE 16        /* its rate, used by the next step's predictor */
+#define CPFFT_W_D 17             /* unrotated strain increment, Voigt 6        */
+#define CPFFT_W_SLIP 23          /* slip increments [nslip]                    */
+#define CPFFT_W_SLIPSUM(ns) (23 + (ns))  /* cumulative slip [nslip]            */
+#define CPFFT_W_WORK(ns) (23 + 2 * (ns)) /* work, plastic work, plastic strain */
+#define CPFFT_NWORDS(ns) (26 + 2 * (ns))
+
+/* ---- context ---------------------------------------------------------- */
+/* FFT_init (FFT_init.f:16-177) + fftAllocate (:191-258) + the per-voxel
+ * orientation setup of setup_mm10_rknstr (drive_eps_sig.f:537-1002).
+ * angles_deg: host array [3][V] of Kocks Euler angles in degrees (the angle
+ * file of inmat_cp); dt: time step (`time step`, indypm.f:56-58).         */
+int cpfft_create(int N, const cpfft_material* mat, const double* angles_deg, double dt,
+                 int gamma_mode, cpfft_ctx** out);
+int cpfft_destroy(cpfft_ctx* ctx);
+const char* cpfft_last_error(void);
+int cpfft_nslip(const cpfft_ctx* ctx);
+/* device address of a context field */
+double* cpfft_field(cpfft_ctx* ctx, int field);
+/* host <-> device copies of whole fields */
+int cpfft_upload(cpfft_ctx* ctx, int field, const double* host);
+int cpfft_download(cpfft_ctx* ctx, int field, double* host);
+/* reset Fn = Fn1 = I, Pn = Pn1 = 0, state = initial (FFT_init.f:153-160) */
+int cpfft_reset(cpfft_ctx* ctx);
+int cpfft_sync(cpfft_ctx* ctx);
+
+/* ---- operators -------------------------------------------------------- */
+/* G_K_dF (G_K_dF.f:11-19): GKF = Gamma:(K4:F) if flgK else Gamma:F.  F and
+ * GKF are device fields [9][V]; may not alias.                            */
+int cpfft_gkdf(cpfft_ctx* ctx, const double* F, double* GKF, int flgK);
+
+/* fftPcg (FFT_nr3.f:214-360): x = CG solution of Gamma:(K4:x) = b, x0 = 0,
+ * stop at ||r|| <= tol*||b|| or ||r|| <= tol, 1000 iterations at most.
+ * Returns 0, 1 (zero right-hand side, x = 0), -1 (tol outside (eps,1)),
+ * -2 (no convergence).                                                    */
+int cpfft_pcg(cpfft_ctx* ctx, const double* b, double* x, double tol, int* iters,
+              double* relres);
+
+/* drive_eps_sig (drive_eps_sig.f:16-45): from FN, FN1 and STATE_N compute
+ * PN1, K4 and STATE_NP1.  iiter = 0 selects the explicit elastic predictor
+ * (rstgp1.f:870-876).  Returns the number of voxels whose local solve
+ * failed after all sub-step cuts (0 normally).                            */
+int cpfft_drive_eps_sig(cpfft_ctx* ctx, int step, int iiter);
+
+/* tangent_homo (tangent_homo.f:11-73): homogenised tangent C_homo(81) on
+ * the host, C_homo[9(j-1)+i] = d mean(P_j) / d Fbar_i.                    */
+int cpfft_tangent_homo(cpfft_ctx* ctx, double tol, double C_homo[81]);
+
+/* update (update.f:16-193) together with the roll-over of FFT_nr3.f:173-175:
+ * state n <- n+1, Fn <- Fn1, Pn <- Pn1.                                   */
+int cpfft_update(cpfft_ctx* ctx);
+
+/* ---- BLAS-1 on device vectors of 9*V doubles (FFT_nr3.f:86-114) ------ */
+int cpfft_dnrm2(cpfft_ctx* ctx, const double* x, double* out);       /* dnrm2 */
+int cpfft_daxpy(cpfft_ctx* ctx, double a, const double* x, double* y); /* daxpy */
+int cpfft_dscal(cpfft_ctx* ctx, double a, double* x);                  /* dscal */
+int cpfft_dcopy(cpfft_ctx* ctx, const double* x, double* y);           /* dcopy */
+/* dFm(1:N3,ii) = DbarF(ii) (FFT_nr3.f:85-87) */
+int cpfft_fill9(cpfft_ctx* ctx, double* x, const double v[9]);
+/* P_bar(ii) = sum_jj Pn1(jj,ii) / N3 (FFT_nr3.f:127-134) */
+int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
+
+/* ---- diagnostics ------------------------------------------------------ */
+/* counters since creation: [0] G_K_dF applications, [1] CG iterations,
+ * [2] drive_eps_sig calls, [3] local predictor iterations, [4] local
+ * Newton iterations, [5] sub-step cuts, [6] failed voxels, [7] residual
+ * evaluations in line searches                                            */
+int cpfft_counters(cpfft_ctx* ctx, long long out[8]);
+/* average device time (ms) of the last timed launches of each kernel
+ * family: [0] G_K_dF, [1] drive_eps_sig, [2] CG vector update            */
+int cpfft_kernel_times(cpfft_ctx* ctx, double out_ms[4]);
+int cpfft_set_timing(cpfft_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

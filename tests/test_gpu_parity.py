@@ -1,0 +1,164 @@
+"""GPU parity: the MI355X operators through the C-ABI against the oracle on
+identical seeded inputs.
+
+Tolerances (FP64):
+  * G_K_dF, mean/norm reductions: 1e-12 relative to the field's max norm
+    (different FFT algorithm and summation order only);
+  * drive_eps_sig per voxel: 1e-9 relative per output field (the local
+    Newton stops at a tolerance, so different rounding can change the last
+    iterate by far less than this; the polar decomposition is evaluated with
+    the reference's operation order on both sides);
+  * CG: identical iteration count, solution 1e-9 relative;
+  * load-step loop: volume-averaged P per step 1e-9 relative, identical
+    Newton / CG / mixed-BC counts.
+"""
+import numpy as np
+import pytest
+
+import cpfft_amd as C
+import oracle_lib as O
+from cpfft_amd.microstructure import polycrystal
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b).max() / max(abs(b).max(), 1e-300)
+
+
+def solver(N, mode, ngr=4, seed=7, mat=None, dt=0.5):
+    _, ang = polycrystal(N, ngr, seed)
+    return C.Solver(N, ang, mat or C.Material(), dt=dt, gamma_mode=mode), ang
+
+
+@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gkdf(N, mode):
+    rng = np.random.default_rng(10 * N + mode)
+    s, _ = solver(N, mode)
+    fo = O.FFT(N, mode)
+    F = rng.standard_normal((9, N ** 3))
+    K4 = rng.standard_normal((81, N ** 3))
+    s.upload(C.DFM, F)
+    s.upload(C.K4, K4)
+    s.gkdf(C.DFM, C.B, 0)
+    assert rel(s.download(C.B), fo.gkdf(F)) < 1e-12
+    s.gkdf(C.DFM, C.B, 1)
+    assert rel(s.download(C.B), fo.gkdf(F, K4)) < 1e-12
+    s.close()
+
+
+def test_blas_and_reductions():
+    N = 8
+    rng = np.random.default_rng(1)
+    s, _ = solver(N, 1)
+    x = rng.standard_normal((9, N ** 3))
+    y = rng.standard_normal((9, N ** 3))
+    s.upload(C.DFM, x)
+    s.upload(C.B, y)
+    import ctypes
+    L = C.lib()
+    out = ctypes.c_double()
+    assert L.cpfft_dnrm2(s.h, s.field(C.DFM), ctypes.byref(out)) == 0
+    assert abs(out.value - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+    L.cpfft_daxpy(s.h, 2.5, s.field(C.DFM), s.field(C.B))
+    assert rel(s.download(C.B), y + 2.5 * x) < 1e-15
+    m = np.zeros(9)
+    L.cpfft_mean9(s.h, s.field(C.DFM), m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert np.allclose(m, x.mean(1), rtol=1e-12, atol=1e-15)
+    v = np.arange(9.0)
+    L.cpfft_fill9(s.h, s.field(C.B), v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert np.array_equal(s.download(C.B), np.repeat(v[:, None], N ** 3, 1))
+    s.close()
+
+
+def _fields_near_identity(rng, V, amp, uniaxial=0.0):
+    F = np.zeros((9, V))
+    F[[0, 4, 8]] = 1.0
+    F += amp * rng.standard_normal((9, V))
+    F[0] += uniaxial
+    F[4] -= 0.5 * uniaxial
+    F[8] -= 0.5 * uniaxial
+    return F
+
+
+@pytest.mark.parametrize("jac_exact", [0, 1])
+def test_drive_eps_sig(jac_exact):
+    """Two load steps of per-voxel updates on 8^3 voxels: elastic predictor,
+    plastic Newton updates, a second step from the first step's state."""
+    N = 8
+    V = N ** 3
+    rng = np.random.default_rng(2 + jac_exact)
+    mat = C.Material(jac_exact=jac_exact)
+    s, ang = solver(N, 1, ngr=8, mat=mat)
+    oc = O.crystal(jac_exact=jac_exact)
+    Fn = _fields_near_identity(rng, V, 0.0)
+    Fn1 = _fields_near_identity(rng, V, 1e-4, uniaxial=4e-4)
+    sn = O.state_init(oc, V)
+    for step, it in [(1, 0), (1, 1), (1, 2)]:
+        s.upload(C.FN, Fn)
+        s.upload(C.FN1, Fn1)
+        s.upload(C.STATE_N, sn)
+        nf = s.drive_eps_sig(step, it)
+        Po, Ko, so, st = O.drive_eps_sig(oc, ang, step, it, 0.5, Fn, Fn1, sn)
+        assert nf == st[3]
+        assert rel(s.download(C.PN1), Po) < 1e-9, (step, it)
+        assert rel(s.download(C.K4), Ko) < 1e-9, (step, it)
+        sg = s.download(C.STATE_NP1)
+        for w0, w1 in [(0, 6), (6, 15), (15, 17), (17, 23), (23, 23 + 2 * oc.nslip),
+                       (23 + 2 * oc.nslip, 26 + 2 * oc.nslip)]:
+            assert rel(sg[w0:w1], so[w0:w1]) < 1e-9, (step, it, w0)
+    # second step from the converged first-step state
+    sn = so
+    Fn = Fn1
+    Fn1 = Fn + _fields_near_identity(rng, V, 5e-5, uniaxial=5e-4) - _fields_near_identity(rng, V, 0.0)
+    s.upload(C.FN, Fn)
+    s.upload(C.FN1, Fn1)
+    s.upload(C.STATE_N, sn)
+    s.drive_eps_sig(2, 1)
+    Po, Ko, so, st = O.drive_eps_sig(oc, ang, 2, 1, 0.5, Fn, Fn1, sn)
+    assert rel(s.download(C.PN1), Po) < 1e-9
+    assert rel(s.download(C.K4), Ko) < 1e-9
+    assert rel(s.download(C.STATE_NP1), so) < 1e-9
+    s.close()
+
+
+@pytest.mark.parametrize("N,mode", [(7, 0), (8, 1)])
+def test_pcg_and_tangent_homo(N, mode):
+    V = N ** 3
+    s, ang = solver(N, mode)
+    oc = O.crystal()
+    I = np.zeros((9, V))
+    I[[0, 4, 8]] = 1.0
+    _, K4, _, _ = O.drive_eps_sig(oc, ang, 1, 0, 0.5, I, I, O.state_init(oc, V))
+    fo = O.FFT(N, mode)
+    dF = np.zeros((9, V))
+    dF[0] = 1e-3
+    b = -fo.gkdf(dF, K4)
+    xo, rco, ito, relo = fo.pcg(K4, b, 1e-8)
+    s.upload(C.K4, K4)
+    s.upload(C.B, b)
+    rc, it, relres = s.pcg(C.B, C.DFM, 1e-8)
+    assert rc == rco == 0 and it == ito
+    assert rel(s.download(C.DFM), xo) < 1e-9
+    Cg = s.tangent_homo(1e-8)
+    Co = fo.tangent_homo(K4, 1e-8)
+    assert rel(Cg, Co) < 1e-9
+    s.close()
+
+
+@pytest.mark.parametrize("N,mode,nstep", [(7, 0, 3), (8, 1, 3)])
+def test_load_steps(N, mode, nstep):
+    """The Fortran driver over the C-ABI against the oracle's FFT_nr3 loop."""
+    wl = C.Workload("t", N, 4, 20261202 + 99, nstep, 0.0005 * nstep, gamma_mode=mode)
+    ang = wl.angles()
+    BC, isNBC = wl.bc()
+    s = C.Solver(N, ang, wl.material, dt=wl.dt, gamma_mode=mode)
+    rc, Pbar, cnt, times = s.compute(BC, isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    oc = O.crystal(jac_exact=wl.material.jac_exact)
+    Pbo, cnto, rco, _ = O.fft_nr3(oc, N, ang, wl.dt, BC, isNBC, wl.tolPCG, wl.tolNR,
+                                   wl.maxIter, gamma_mode=mode)
+    assert rc == 0 and rco == 0
+    assert np.array_equal(cnt[:, [0, 2, 3]], cnto[:, [0, 2, 3]])
+    assert rel(Pbar, Pbo) < 1e-9
+    s.close()
