@@ -1,0 +1,188 @@
+"""Benchmark: converged load steps per second of the finite-strain FFT
+crystal-plasticity solver on one MI355X.
+
+A step is one converged load step of the FFT_nr3 loop (mixed-BC uniaxial
+tension of the 128^3, 256-grain FCC polycrystal), driven by the Fortran
+driver over the HIP operators.  W warm-up steps run first (they include the
+initial elastic call and homogenised tangent), then K steps are timed.
+
+Prints one JSON line.  The roofline object describes G_K_dF with K4 (the
+CG matrix-vector product, five kernels launched as one group), whose device
+time is measured with HIP events on the solver's stream inside the timed
+region.  The cpu_baseline object times the oracle (the CPU restatement, all
+host cores) on a bounded sample of the same workload and scales it by this
+run's operator counts per load step.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import cpfft_amd as C  # noqa: E402
+from cpfft_amd.configs import WORKLOADS  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per voxel of one G_K_dF with K4: read dF (72) + K4 (648),
+# write the result (72)
+GKDF_BYTES_PER_VOXEL = 792.0
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(wl, solver, per_step, budget_s=20.0):
+    """Time the oracle on a bounded sample and scale it to one load step:
+    (G_K_dF applications per step) x t(G_K_dF) + (drive_eps_sig calls per
+    step) x t(drive_eps_sig over all voxels, from a voxel sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    V = wl.N ** 3
+    # operator sample: one G_K_dF with K4 at full size on the current state
+    K4 = solver.download(C.K4)
+    F = solver.download(C.DFM)
+    fo = O.FFT(wl.N, wl.gamma_mode)
+    t = time.perf_counter()
+    fo.gkdf(F, K4)
+    t_gkdf = time.perf_counter() - t
+    del K4
+    # material sample: a contiguous slab of voxels at the current state,
+    # plastic update (Newton iteration 1)
+    nv = min(V, 16384)
+    sl = slice(V // 2, V // 2 + nv)
+    Fn = np.ascontiguousarray(solver.download(C.FN)[:, sl])
+    Fn1 = np.ascontiguousarray(solver.download(C.FN1)[:, sl])
+    sn = np.ascontiguousarray(solver.download(C.STATE_N)[:, sl])
+    ang = np.ascontiguousarray(wl.angles()[:, sl])
+    oc = O.crystal(jac_exact=wl.material.jac_exact)
+    t = time.perf_counter()
+    O.drive_eps_sig(oc, ang, 3, 1, wl.dt, Fn, Fn1, sn)
+    t_des = (time.perf_counter() - t) * (V / nv)
+    n_gkdf, n_des = per_step
+    step_s = n_gkdf * t_gkdf + n_des * t_des
+    return {
+        "value": 1.0 / step_s,
+        "unit": "load steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle G_K_dF with K4 once at {wl.N}^3 ({t_gkdf:.2f} s) and oracle "
+                   f"drive_eps_sig on {nv} voxels of the current {wl.N}^3 state "
+                   f"({t_des * nv / V:.2f} s, x{V / nv:.0f} for all voxels); one load step = "
+                   f"{n_gkdf:.0f} G_K_dF + {n_des:.1f} drive_eps_sig calls (this run's counts)"),
+        "seconds_per_step": step_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3_128")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    wl = WORKLOADS[args.workload]
+    W, K = args.warmup, args.steps
+    nstep = W + K
+    BC_all, isNBC = wl.bc()
+    if nstep > BC_all.shape[0]:
+        raise SystemExit(f"{args.workload} has {BC_all.shape[0]} load steps")
+    ang = wl.angles()
+    s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    print(f"[bench] {wl.name}: N={wl.N} grains={wl.ngrains} warmup={W} steps={K}", flush=True)
+
+    rc, Pw, cw, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1)
+    if rc != 0:
+        raise SystemExit(f"warm-up failed with code {rc}")
+    print(f"[bench] warm-up done: counters {cw.tolist()}", flush=True)
+    c0 = s.counters()
+    s.set_timing(True)
+    device_sync()
+    s.sync()
+    t0 = time.perf_counter()
+    rc, Pbar, cnt, times = s.compute(BC_all[W:nstep], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter,
+                                     first_step=W + 1)
+    s.sync()
+    device_sync()
+    t1 = time.perf_counter()
+    if rc != 0:
+        raise SystemExit(f"timed steps failed with code {rc}")
+    avg, nlaunch = s.kernel_times()
+    c1 = s.counters()
+    wall = t1 - t0
+    V = wl.N ** 3
+    d = [b - a for a, b in zip(c0, c1)]
+    n_des = d[2]
+    n_gkdf = d[0]
+    steps_per_s = K / wall
+    gk_ms = avg[0]
+    achieved = GKDF_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
+    out = {
+        "metric": "converged load steps/s at 128^3 (plain method)",
+        "value": steps_per_s,
+        "unit": "load steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{wl.name}: {wl.N}^3 periodic Voronoi, {wl.ngrains} grains, FCC cubic "
+                         f"Voce n=20, uniaxial tension 1e-3/s, dt={wl.dt}s, steps {W + 1}-{nstep}"),
+            "grid": wl.N, "grains": wl.ngrains, "gamma_mode": wl.gamma_mode,
+            "jac_exact": wl.material.jac_exact, "tolPCG": wl.tolPCG, "tolNR": wl.tolNR,
+        },
+        "voxel_updates_per_s": V * n_des / wall,
+        "per_step": {
+            "G_K_dF": n_gkdf / K, "cg_iterations": d[1] / K, "drive_eps_sig": n_des / K,
+            "newton": float(cnt[:, 0].mean()), "mixed_bc_loops": float(cnt[:, 2].mean()),
+            "local_newton_iters_per_voxel_call": d[4] / max(1, n_des * V),
+            "local_failures": d[6],
+        },
+        "category_s": {"pcg": float(times[0]), "sig_eps": float(times[1]), "total": float(times[2])},
+        "kernel_ms": {"G_K_dF_with_K4": avg[0], "drive_eps_sig": avg[1], "cg_update": avg[2],
+                      "G_K_dF_no_K4": avg[3]},
+        "roofline": {
+            "kernel": "G_K_dF with K4 (5-kernel launch group)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": GKDF_BYTES_PER_VOXEL * V,
+            "avg_launch_ms": gk_ms,
+            "launches_measured": int(nlaunch[0]),
+        },
+        "Pbar_xx_last": float(Pbar[-1, 0]),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, s, (n_gkdf / K, n_des / K))
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,7 +94,7 @@ def driver():
         if not os.path.exists(DRIVER):
             raise RuntimeError(f"{DRIVER} is missing: build it first (python -m cpfft_amd.build)")
         D = ctypes.CDLL(DRIVER)
-        D.cpfft_compute.argtypes = [_vp, ctypes.c_int, _dp, _ip, ctypes.c_double,
+        D.cpfft_compute.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _ip, _dp]
         _drv = D
     return _drv
@@ -196,13 +196,18 @@ class Solver:
         lib().cpfft_set_timing(self.h, int(on))
 
     def kernel_times(self):
-        out = np.zeros(4)
+        """(average ms per launch group, counts) for: G_K_dF with K4,
+        drive_eps_sig, CG vector update, G_K_dF without K4"""
+        out = np.zeros(8)
         lib().cpfft_kernel_times(self.h, _p(out))
-        return out
+        return out[:4], out[4:].astype(int)
 
-    def compute(self, BC_all, isNBC, tolPCG=1e-8, tolNR=1e-6, maxIter=25, verbose=False):
+    def compute(self, BC_all, isNBC, tolPCG=1e-8, tolNR=1e-6, maxIter=25, verbose=False,
+                first_step=1):
         """The `compute` command: drive_eps_sig(1,0) then the FFT_nr3 load-step
-        loop, run by the Fortran driver.  Returns (rc, Pbar[nstep,9],
+        loop, run by the Fortran driver, for the steps whose boundary values
+        BC_all holds, numbered from first_step (a first_step above 1
+        continues the previous call).  Returns (rc, Pbar[nstep,9],
         counters[nstep,4], times[3])."""
         BC = np.ascontiguousarray(BC_all, dtype=np.float64)
         nstep = BC.shape[0]
@@ -210,7 +215,8 @@ class Solver:
         Pbar = np.zeros((nstep, 9))
         cnt = np.zeros((nstep, 4), dtype=np.int32)
         times = np.zeros(3)
-        rc = driver().cpfft_compute(self.h, nstep, _p(BC), nbc.ctypes.data_as(_ip), tolPCG, tolNR,
+        rc = driver().cpfft_compute(self.h, first_step, nstep, _p(BC), nbc.ctypes.data_as(_ip),
+                                    tolPCG, tolNR,
                                     maxIter, int(verbose), _p(Pbar), cnt.ctypes.data_as(_ip),
                                     _p(times))
         return rc, Pbar, cnt, times

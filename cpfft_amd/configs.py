@@ -9,8 +9,13 @@ every other component of F prescribed with zero increment.
 
 Time stepping: the FCC material (rate exponent 20, Voce tau_y = 16 MPa,
 theta_0 = 250 MPa, reference slip rate 1e-3/s) is loaded at 1e-3/s with a
-0.5 s step (5e-4 strain per step).  With a 1 s step the reference's global
-Newton iteration (no line search) does not converge for this material.
+0.25 s step (2.5e-4 strain per step).  The reference's global Newton
+iteration (FFT_nr3.f:99-123: full steps, no line search) does not converge
+on these microstructures with coarser steps: with 1 s steps the local solves
+exhaust their sub-step cuts in step 1, and with 0.5 s steps voxels at the
+onset of yield flip between plastic loading and elastic unloading on every
+Newton iteration at 64^3 (a period-2 cycle of the correction, whose norm
+then stays fixed at ~2.3e-6 above tolNR = 1e-6).
 """
 from dataclasses import dataclass, field
 
@@ -54,7 +59,7 @@ class Workload:
     seed: int
     nstep: int
     strain: float               # final F_xx - 1
-    dt: float = 0.5
+    dt: float = 0.25
     tolPCG: float = 1.0e-8
     tolNR: float = 1.0e-6
     maxIter: int = 25
@@ -80,7 +85,7 @@ class Workload:
 
 # seeds: numpy default_rng(20261202 + k) for workload k
 WORKLOADS = {
-    "c1_16": Workload("c1_16", 16, 8, 20261202 + 1, 40, 0.02),
-    "c2_64": Workload("c2_64", 64, 64, 20261202 + 2, 80, 0.04),
-    "c3_128": Workload("c3_128", 128, 256, 20261202 + 3, 120, 0.06),
+    "c1_16": Workload("c1_16", 16, 8, 20261202 + 1, 80, 0.02),
+    "c2_64": Workload("c2_64", 64, 64, 20261202 + 2, 160, 0.04),
+    "c3_128": Workload("c3_128", 128, 256, 20261202 + 3, 240, 0.06),
 }

@@ -55,11 +55,15 @@ struct cpfft_ctx {
   int32_t* oid = nullptr;
   long long* dcnt = nullptr;
   long long cg_iters = 0, gkdf_calls = 0, des_calls = 0;
+  // kernel timing: event pairs recorded on the stream without host
+  // synchronisation, resolved by cpfft_kernel_times
+  static constexpr int kFam = 4, kRing = 8192;
   int timing = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double t_ms[4] = {0, 0, 0, 0};
-  long t_n[4] = {0, 0, 0, 0};
-  double t_last[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev[kFam][2];
+  int ev_n[kFam] = {0, 0, 0, 0};
+  double t_ms[kFam] = {0, 0, 0, 0};
+  long t_n[kFam] = {0, 0, 0, 0};
+  int open_fam = -1;
 };
 
 // ------------------------------------------------------------------ setup
@@ -201,8 +205,9 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->hres) hipHostFree(c->hres);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
+  for (int f = 0; f < cpfft_ctx::kFam; ++f)
+    for (int k = 0; k < 2; ++k)
+      for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -325,8 +330,6 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   }
   HIP_OK(hipMalloc(&c->tw, N * sizeof(cplx)));
   HIP_OK(hipMemcpy(c->tw, htw.data(), N * sizeof(cplx), hipMemcpyHostToDevice));
-  HIP_OK(hipEventCreate(&c->ev0));
-  HIP_OK(hipEventCreate(&c->ev1));
   rc = cpfft_reset(c);
   if (rc) { cpfft_destroy(c); return rc; }
   *out = c;
@@ -384,27 +387,56 @@ extern "C" int cpfft_reset(cpfft_ctx* c) {
 }
 
 // ------------------------------------------------------------ timing
-static void tstart(cpfft_ctx* c) {
-  if (c->timing) hipEventRecord(c->ev0, c->stream);
+// Families: 0 G_K_dF with K4, 1 drive_eps_sig, 2 CG vector update,
+// 3 G_K_dF without K4.
+static void flush_events(cpfft_ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  for (int f = 0; f < cpfft_ctx::kFam; ++f) {
+    for (int k = 0; k < c->ev_n[f]; ++k) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev[f][0][k], c->ev[f][1][k]) == hipSuccess) {
+        c->t_ms[f] += ms;
+        c->t_n[f] += 1;
+      }
+    }
+    c->ev_n[f] = 0;
+  }
+}
+static void tstart(cpfft_ctx* c, int fam) {
+  if (!c->timing) return;
+  if (c->ev_n[fam] == cpfft_ctx::kRing) flush_events(c);
+  int k = c->ev_n[fam];
+  if ((int)c->ev[fam][0].size() <= k) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    c->ev[fam][0].push_back(a);
+    c->ev[fam][1].push_back(b);
+  }
+  (void)hipEventRecord(c->ev[fam][0][k], c->stream);
+  c->open_fam = fam;
 }
 static void tstop(cpfft_ctx* c, int fam) {
-  if (!c->timing) return;
-  hipEventRecord(c->ev1, c->stream);
-  hipEventSynchronize(c->ev1);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
-  c->t_ms[fam] += ms;
-  c->t_n[fam] += 1;
+  if (!c->timing || c->open_fam != fam) return;
+  (void)hipEventRecord(c->ev[fam][1][c->ev_n[fam]], c->stream);
+  c->ev_n[fam]++;
+  c->open_fam = -1;
 }
 
 extern "C" int cpfft_set_timing(cpfft_ctx* c, int on) {
+  flush_events(c);
   c->timing = on;
-  for (int i = 0; i < 4; ++i) { c->t_ms[i] = 0; c->t_n[i] = 0; }
+  for (int i = 0; i < cpfft_ctx::kFam; ++i) { c->t_ms[i] = 0; c->t_n[i] = 0; }
   return 0;
 }
 
-extern "C" int cpfft_kernel_times(cpfft_ctx* c, double out[4]) {
-  for (int i = 0; i < 4; ++i) out[i] = c->t_n[i] ? c->t_ms[i] / c->t_n[i] : 0.0;
+// average device time (ms) per launch group of each family since
+// cpfft_set_timing, followed by the counts in out[4..7]
+extern "C" int cpfft_kernel_times(cpfft_ctx* c, double out[8]) {
+  flush_events(c);
+  for (int i = 0; i < cpfft_ctx::kFam; ++i) {
+    out[i] = c->t_n[i] ? c->t_ms[i] / c->t_n[i] : 0.0;
+    out[4 + i] = (double)c->t_n[i];
+  }
   return 0;
 }
 
@@ -414,9 +446,10 @@ static int check_ptr(const void* p) { return p ? 0 : fail(-1, "null device point
 extern "C" int cpfft_gkdf(cpfft_ctx* c, const double* F, double* GKF, int flgK) {
   if (!c || check_ptr(F) || check_ptr(GKF)) return -1;
   if (F == GKF) return fail(-1, "cpfft_gkdf: F and GKF must not alias");
-  tstart(c);
+  const int fam = flgK ? 0 : 3;
+  tstart(c, fam);
   launch_gkdf(c->geom, F, flgK ? c->fields[CPFFT_K4] : nullptr, GKF, c->spec, c->tw, c->stream);
-  tstop(c, 0);
+  tstop(c, fam);
   HIP_OK(hipGetLastError());
   c->gkdf_calls++;
   return 0;
@@ -492,14 +525,14 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   double rr = bb, resnorm = n2b;
   int k;
   for (k = 1; k <= maxIter; ++k) {
-    tstart(c);
+    tstart(c, 0);
     launch_gkdf(c->geom, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
     tstop(c, 0);
     c->gkdf_calls++;
     double pq;
     if ((rc = dot_host(c, c->p, c->q, &pq))) return rc;
     double alpha = rr / pq;
-    tstart(c);
+    tstart(c, 2);
     launch_cg_update(n, alpha, c->p, c->q, x, c->r, c->partials, c->dres, c->stream);
     tstop(c, 2);
     if ((rc = read_res(c, 1))) return rc;
@@ -521,8 +554,8 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
 
 extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
   long long before[8];
-  HIP_OK(hipMemcpy(before, c->dcnt, sizeof before, hipMemcpyDeviceToHost));
-  tstart(c);
+  HIP_OK(hipMemcpyAsync(before, c->dcnt, sizeof before, hipMemcpyDeviceToHost, c->stream));
+  tstart(c, 1);
   if (c->nslip == 12)
     launch_drive_eps_sig<12>(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
@@ -549,9 +582,9 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
   double* b = c->tmp9;
   for (int i = 0; i < 9; ++i) {
     launch_gather_k4_col(V, i, c->fields[CPFFT_K4], C, c->stream);
-    tstart(c);
+    tstart(c, 3);
     launch_gkdf(c->geom, C, nullptr, b, c->spec, c->tw, c->stream);
-    tstop(c, 0);
+    tstop(c, 3);
     c->gkdf_calls++;
     launch_scal(9 * V, -1.0, b, c->stream);
     int it;

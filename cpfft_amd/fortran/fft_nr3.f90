@@ -16,6 +16,12 @@ module cpfft_driver
   private
   public :: cpfft_compute
 
+  ! state carried from one load step to the next (FFT_nr3.f:23-34 keeps
+  ! these as saved locals of the routine)
+  real(c_double), save :: barF_t(9) = [1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0]
+  real(c_double), save :: P_bar(9) = 0.0d0
+  real(c_double), save :: C_homo(81) = 0.0d0
+
 contains
 
   ! NBC_update (FFT_nr3.f:375-433): rows of prescribed P take the
@@ -79,7 +85,11 @@ contains
     wall = dble(cnt) / dble(rate)
   end function wall
 
-  ! compute + FFT_nr3.  BC_all(9,nstep) as filled by inlod (inlod.f:57-63);
+  ! compute + FFT_nr3 for load steps first_step .. first_step+nstep-1.
+  ! first_step = 1 starts a run (elastic call and initial tangent); a later
+  ! first_step continues the previous run on the same context.
+  ! BC_all(9,nstep) holds those steps' boundary values as inlod fills them
+  ! (inlod.f:57-63);
   ! returns 0 or the negative code of the reference's die_abort paths:
   ! -2 CG not converged (FFT_nr3.f:335-339), -3 NBC_update singular
   ! (:418-421), -4 Newton maxIter (:116-119), -5 mixed-BC maxIter
@@ -87,20 +97,20 @@ contains
   ! counters(:,step) = Newton iterations, CG iterations, mixed-BC loops,
   ! drive_eps_sig calls; times = pcg, sig-eps, total wall seconds (thyme
   ! categories 1 and 2, thyme.f:15-47).
-  integer(c_int) function cpfft_compute(ctx, nstep, BC_all, isNBC_c, tolPCG, tolNR, maxIter, &
-       verbose, Pbar_out, counters, times) bind(C, name="cpfft_compute")
+  integer(c_int) function cpfft_compute(ctx, first_step, nstep, BC_all, isNBC_c, tolPCG, tolNR, &
+       maxIter, verbose, Pbar_out, counters, times) bind(C, name="cpfft_compute")
     type(c_ptr), value     :: ctx
-    integer(c_int), value  :: nstep, maxIter, verbose
+    integer(c_int), value  :: first_step, nstep, maxIter, verbose
     real(c_double), value  :: tolPCG, tolNR
     real(c_double)         :: BC_all(9, nstep), Pbar_out(9, nstep), times(3)
     integer(c_int)         :: isNBC_c(9), counters(4, nstep)
 
     type(c_ptr) :: Fn1, Pn1, dFm, b
-    real(c_double) :: P_bar(9), resPbar(3), DbarF(9), barF(9), barF_t(9)
-    real(c_double) :: FBC(9), PBC(9), C_homo(81), Fnorm, resfft, relres, dn, t0, t1
+    real(c_double) :: resPbar(3), DbarF(9), barF(9)
+    real(c_double) :: FBC(9), PBC(9), Fnorm, resfft, relres, dn, t0, t1
     integer(c_int) :: rc, its
     integer(c_long_long) :: cnt0(8), cnt1(8)
-    integer :: step, ii, iiter_EBC, iiter_NBC, info, nnewton, ncalls, nloops
+    integer :: step, istep, ii, iiter_EBC, iiter_NBC, info, nnewton, ncalls, nloops
     logical :: isNBC(9), existNBC
 
     cpfft_compute = 0
@@ -109,33 +119,36 @@ contains
     Pbar_out = 0.0d0
     isNBC = isNBC_c /= 0
     existNBC = any(isNBC)
-    barF_t = [1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0]
-    P_bar = 0.0d0
     Fn1 = cpfft_field(ctx, CPFFT_FN1)
     Pn1 = cpfft_field(ctx, CPFFT_PN1)
     dFm = cpfft_field(ctx, CPFFT_DFM)
     b = cpfft_field(ctx, CPFFT_B)
     times(3) = wall()
 
-    ! compute (FFT_finite_3d.f:145): elastic tangent at F = I
-    t0 = wall()
-    rc = cpfft_drive_eps_sig(ctx, 1, 0)
-    if (rc < 0) then
-      cpfft_compute = rc
-      return
-    end if
-    times(2) = times(2) + (wall() - t0)
+    if (first_step <= 1) then
+      barF_t = [1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0]
+      P_bar = 0.0d0
+      ! compute (FFT_finite_3d.f:145): elastic tangent at F = I
+      t0 = wall()
+      rc = cpfft_drive_eps_sig(ctx, 1, 0)
+      if (rc < 0) then
+        cpfft_compute = rc
+        return
+      end if
+      times(2) = times(2) + (wall() - t0)
 
-    ! initial homogenised tangent (FFT_nr3.f:47)
-    t0 = wall()
-    rc = cpfft_tangent_homo(ctx, tolPCG, C_homo)
-    times(1) = times(1) + (wall() - t0)
-    if (rc < 0) then
-      cpfft_compute = rc
-      return
+      ! initial homogenised tangent (FFT_nr3.f:47)
+      t0 = wall()
+      rc = cpfft_tangent_homo(ctx, tolPCG, C_homo)
+      times(1) = times(1) + (wall() - t0)
+      if (rc < 0) then
+        cpfft_compute = rc
+        return
+      end if
     end if
 
-    do step = 1, nstep
+    do istep = 1, nstep
+      step = max(first_step, 1) + istep - 1
       if (verbose /= 0) write(*, 1000) step
       rc = cpfft_counters(ctx, cnt0)
       nnewton = 0
@@ -146,9 +159,9 @@ contains
       DbarF = 0.0d0
       do ii = 1, 9
         if (isNBC(ii)) then
-          PBC(ii) = BC_all(ii, step)
+          PBC(ii) = BC_all(ii, istep)
         else
-          FBC(ii) = BC_all(ii, step)
+          FBC(ii) = BC_all(ii, istep)
           DbarF(ii) = FBC(ii) - barF_t(ii)
         end if
       end do
@@ -264,12 +277,12 @@ contains
       ! roll-over (FFT_nr3.f:173-179)
       barF_t(1:9) = barF(1:9)
       rc = cpfft_update(ctx)
-      Pbar_out(:, step) = P_bar
+      Pbar_out(:, istep) = P_bar
       rc = cpfft_counters(ctx, cnt1)
-      counters(1, step) = nnewton
-      counters(2, step) = int(cnt1(2) - cnt0(2))
-      counters(3, step) = nloops
-      counters(4, step) = ncalls
+      counters(1, istep) = nnewton
+      counters(2, istep) = int(cnt1(2) - cnt0(2))
+      counters(3, istep) = nloops
+      counters(4, istep) = ncalls
     end do
     rc = cpfft_sync(ctx)
     times(3) = wall() - times(3)

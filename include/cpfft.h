@@ -142,9 +142,12 @@ int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
  * Newton iterations, [5] sub-step cuts, [6] failed voxels, [7] residual
  * evaluations in line searches                                            */
 int cpfft_counters(cpfft_ctx* ctx, long long out[8]);
-/* average device time (ms) of the last timed launches of each kernel
- * family: [0] G_K_dF, [1] drive_eps_sig, [2] CG vector update            */
-int cpfft_kernel_times(cpfft_ctx* ctx, double out_ms[4]);
+/* device-time measurement with HIP events recorded on the context's stream
+ * (no host synchronisation while recording).  out[0..3]: average ms per
+ * launch group since cpfft_set_timing of [0] G_K_dF with K4 (five kernels),
+ * [1] drive_eps_sig, [2] CG vector update, [3] G_K_dF without K4;
+ * out[4..7]: number of launch groups measured.                           */
+int cpfft_kernel_times(cpfft_ctx* ctx, double out[8]);
 int cpfft_set_timing(cpfft_ctx* ctx, int on);
 
 #ifdef __cplusplus

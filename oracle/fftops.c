@@ -128,15 +128,23 @@ static void dft1(const orc_fft* f, double* xr, double* xi, long stride, int sign
   }
 }
 
-static void dft3(orc_fft* f, double* re, double* im, int sign) {
+/* 3-D DFT as three passes of 1-D transforms (lines in parallel) */
+static void dft3(const orc_fft* f, double* re, double* im, int sign) {
   int N = f->N;
   long NN = (long)N * N;
-  double* wr = f->line_r;
-  double* wi = f->line_i;
-  for (long a = 0; a < NN; ++a) dft1(f, re + a * N, im + a * N, 1, sign, wr, wi);
-  for (int k = 0; k < N; ++k)
-    for (int i = 0; i < N; ++i) dft1(f, re + k * NN + i, im + k * NN + i, N, sign, wr, wi);
-  for (long a = 0; a < NN; ++a) dft1(f, re + a, im + a, NN, sign, wr, wi);
+#pragma omp parallel
+  {
+    double wr[256], wi[256];
+#pragma omp for schedule(static)
+    for (long a = 0; a < NN; ++a) dft1(f, re + a * N, im + a * N, 1, sign, wr, wi);
+#pragma omp for schedule(static)
+    for (long a = 0; a < NN; ++a) {
+      long k = a / N, i = a % N;
+      dft1(f, re + k * NN + i, im + k * NN + i, N, sign, wr, wi);
+    }
+#pragma omp for schedule(static)
+    for (long a = 0; a < NN; ++a) dft1(f, re + a, im + a, NN, sign, wr, wi);
+  }
 }
 
 /* ddot42n (G_K_dF.f:241-268) for one voxel with stride */
@@ -182,12 +190,14 @@ void orc_gkdf(orc_fft* f, const double* K4, const double* F, double* GKF, int fl
     double* r = re + c * V;
     double* i = im + c * V;
     const double* x = GKF + c * V;
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < V; ++e) { r[e] = x[e] * f->c1[e]; i[e] = x[e] * f->c2[e]; }
     dft3(f, r, i, -1);
   }
   /* Ghat contraction on re and im */
-  double g[81];
+#pragma omp parallel for schedule(static)
   for (long e = 0; e < V; ++e) {
+    double g[81];
     ghat_voxel(f, e, g);
     double rr[9], ri[9];
     for (int row = 0; row < 9; ++row) {
@@ -203,6 +213,7 @@ void orc_gkdf(orc_fft* f, const double* K4, const double* F, double* GKF, int fl
     double* i = im + c * V;
     dft3(f, r, i, +1);
     double* y = GKF + c * V;
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < V; ++e) {
       double a = (r[e] * sc) * f->c1[e];
       double b = (i[e] * sc) * f->c2[e];
