@@ -184,6 +184,31 @@ class Solver:
     def update(self):
         _check(lib().cpfft_update(self.h), "cpfft_update")
 
+    # BLAS-1 on whole [9][V] fields (the MKL calls of FFT_nr3.f:86-147)
+    def dnrm2(self, f):
+        out = ctypes.c_double(0.0)
+        _check(lib().cpfft_dnrm2(self.h, self.field(f), ctypes.byref(out)), "cpfft_dnrm2")
+        return out.value
+
+    def daxpy(self, a, x, y):
+        _check(lib().cpfft_daxpy(self.h, a, self.field(x), self.field(y)), "cpfft_daxpy")
+
+    def dscal(self, a, x):
+        _check(lib().cpfft_dscal(self.h, a, self.field(x)), "cpfft_dscal")
+
+    def dcopy(self, x, y):
+        _check(lib().cpfft_dcopy(self.h, self.field(x), self.field(y)), "cpfft_dcopy")
+
+    def fill9(self, x, v):
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        assert vv.size == 9
+        _check(lib().cpfft_fill9(self.h, self.field(x), _p(vv)), "cpfft_fill9")
+
+    def mean9(self, x):
+        out = np.zeros(9)
+        _check(lib().cpfft_mean9(self.h, self.field(x), _p(out)), "cpfft_mean9")
+        return out
+
     def sync(self):
         _check(lib().cpfft_sync(self.h), "cpfft_sync")
 

@@ -311,7 +311,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
     c->field_len[f] = lens[f];
     HIP_OK(hipMalloc(&c->fields[f], lens[f] * sizeof(double)));
   }
-  HIP_OK(hipMalloc(&c->spec, 9 * g.S1 * sizeof(cplx)));
+  HIP_OK(hipMalloc(&c->spec, 2 * 9 * g.S1 * sizeof(cplx)));
   HIP_OK(hipMalloc(&c->r, 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->p, 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->q, 9 * V * sizeof(double)));

@@ -16,6 +16,7 @@
 //    block reductions (fixed order, no floating-point atomics).
 //  * drive_eps_sig (src/drive_eps_sig.f:16-339): one thread per voxel,
 //    body in mm10_core.h.
+#include "green.h"
 #include "kernels.h"
 
 namespace cpfft {
@@ -23,11 +24,11 @@ namespace cpfft {
 static constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ FFT
-// Twiddle for exponent e (mod N) of the N-th root table, conjugated for
-// the inverse transform.
-__device__ inline cplx tw_get(const cplx* __restrict__ tw, int N, long e, double sgn) {
-  int k = (int)(e % N);
-  cplx w = tw[k];
+// Twiddle for exponent e (0 <= e < N: every caller's exponent t*k*N/(Ns*r)
+// with t < r, k < Ns, or ((q t) mod r) * N/r, stays below N) of the N-th
+// root table, conjugated for the inverse transform.
+__device__ inline cplx tw_get(const cplx* __restrict__ tw, int, int e, double sgn) {
+  cplx w = tw[e];
   w.im *= sgn;  // table holds exp(-2 pi i k/N); sgn=+1 forward, -1 inverse
   return w;
 }
@@ -60,14 +61,14 @@ __device__ void lds_fft(cplx* buf, cplx* tmp, int nlines, int ls, int es, const 
       const int o = (j / Ns) * Ns * r + k;
       if (r == 2) {
         cplx a0 = s[j * es];
-        cplx a1 = cmul(s[(j + nb) * es], tw_get(tw, N, (long)k * tstride, fwd));
+        cplx a1 = cmul(s[(j + nb) * es], tw_get(tw, N, k * tstride, fwd));
         d[o * es] = cplx{a0.re + a1.re, a0.im + a1.im};
         d[(o + Ns) * es] = cplx{a0.re - a1.re, a0.im - a1.im};
       } else if (r == 4) {
         cplx a0 = s[j * es];
-        cplx a1 = cmul(s[(j + nb) * es], tw_get(tw, N, (long)k * tstride, fwd));
-        cplx a2 = cmul(s[(j + 2 * nb) * es], tw_get(tw, N, (long)2 * k * tstride, fwd));
-        cplx a3 = cmul(s[(j + 3 * nb) * es], tw_get(tw, N, (long)3 * k * tstride, fwd));
+        cplx a1 = cmul(s[(j + nb) * es], tw_get(tw, N, k * tstride, fwd));
+        cplx a2 = cmul(s[(j + 2 * nb) * es], tw_get(tw, N, 2 * k * tstride, fwd));
+        cplx a3 = cmul(s[(j + 3 * nb) * es], tw_get(tw, N, 3 * k * tstride, fwd));
         cplx b0{a0.re + a2.re, a0.im + a2.im}, b1{a0.re - a2.re, a0.im - a2.im};
         cplx b2{a1.re + a3.re, a1.im + a3.im}, b3{a1.re - a3.re, a1.im - a3.im};
         // multiply b3 by -i (forward) or +i (inverse)
@@ -83,7 +84,7 @@ __device__ void lds_fft(cplx* buf, cplx* tmp, int nlines, int ls, int es, const 
         for (int t = 0; t < 16; ++t) {
           if (t < r) {
             cplx v = s[(j + t * nb) * es];
-            x[t] = t ? cmul(v, tw_get(tw, N, (long)t * k * tstride, fwd)) : v;
+            x[t] = t ? cmul(v, tw_get(tw, N, t * k * tstride, fwd)) : v;
           }
         }
         const int rstep = N / r;  // r-th roots from the N-th root table
@@ -94,7 +95,7 @@ __device__ void lds_fft(cplx* buf, cplx* tmp, int nlines, int ls, int es, const 
 #pragma unroll
             for (int t = 1; t < 16; ++t) {
               if (t < r) {
-                cplx w = tw_get(tw, N, (long)((q * t) % r) * rstep, fwd);
+                cplx w = tw_get(tw, N, ((q * t) % r) * rstep, fwd);
                 cplx m = cmul(x[t], w);
                 acc.re += m.re;
                 acc.im += m.im;
@@ -184,30 +185,6 @@ __global__ __launch_bounds__(kBlock) void k_fft_y(SpecGeom g, cplx* __restrict__
   }
 }
 
-// Green operator on one frequency: out_ij = q_j (A q)_i / |q|^2, added with weight w
-__device__ inline void gamma_acc(const double q[3], const cplx A[9], cplx out[9], double w) {
-  double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-  if (qq <= 1e-10) return;
-  double s = w / qq;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    cplx aq{A[3 * i].re * q[0] + A[3 * i + 1].re * q[1] + A[3 * i + 2].re * q[2],
-            A[3 * i].im * q[0] + A[3 * i + 1].im * q[1] + A[3 * i + 2].im * q[2]};
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      out[3 * i + j].re += aq.re * q[j] * s;
-      out[3 * i + j].im += aq.im * q[j] * s;
-    }
-  }
-}
-
-__device__ inline int wrapf(int v, int N) {
-  // representative in [-N/2, N/2) for even N, (-N/2, N/2] for odd N
-  int h = N / 2;
-  int m = ((v + h) % N + N) % N;
-  return m - h;
-}
-
 // pass 3: x transforms + projection. grid: (ceil(Nz/tz), N)
 __global__ __launch_bounds__(kBlock) void k_proj_x(SpecGeom g, cplx* __restrict__ spec,
                                                    const cplx* __restrict__ tw) {
@@ -229,35 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_proj_x(SpecGeom g, cplx* __restrict_
   __syncthreads();
   lds_fft(buf, tmp, nl, N, 1, g.plan, tw, 1.0);
   // projection at every (x, kz) point of the tile
-  const bool even = (N % 2) == 0;
   for (int i = threadIdx.x; i < N * nt; i += blockDim.x) {
     int x = i / nt, t = i - x * nt;
-    int f[3] = {x <= N / 2 ? x : x - N, y <= N / 2 ? y : y - N, kz0 + t};
     cplx A[9], out[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      A[c] = buf[(c * nt + t) * N + x];
-      out[c] = cplx{0.0, 0.0};
-    }
-    if (g.gamma_mode == 0 && even) {
-      // reference operator: M(f) = (Gamma(w(f-1)) + Gamma(w(-f-1)))/2
-      double q1[3], q2[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        q1[a] = (double)wrapf(f[a] - 1, N);
-        q2[a] = (double)wrapf(-f[a] - 1, N);
-      }
-      gamma_acc(q1, A, out, 0.5);
-      gamma_acc(q2, A, out, 0.5);
-    } else {
-      bool nyq = even && (abs(f[0]) == N / 2 || abs(f[1]) == N / 2 || f[2] == N / 2);
-      if (!nyq) {
-        double q[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) q[a] = (double)wrapf(f[a], N);
-        gamma_acc(q, A, out, 1.0);
-      }
-    }
+    for (int c = 0; c < 9; ++c) A[c] = buf[(c * nt + t) * N + x];
+    project9(N, g.gamma_mode, x, y, kz0 + t, A, out);
 #pragma unroll
     for (int c = 0; c < 9; ++c) buf[(c * nt + t) * N + x] = out[c];
   }
@@ -303,6 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_inv(SpecGeom g, const cplx* __
 
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
                  const cplx* tw, hipStream_t s) {
+  if (launch_gkdf_pencil(g, F, K4, out, spec, tw, s)) return;
   const int N = g.N;
   size_t lz = 2 * 9 * (size_t)N * sizeof(cplx);
   k_fft_z_fwd<<<dim3((unsigned)((long)N * N)), kBlock, lz, s>>>(g, F, K4, spec, tw);

@@ -1,0 +1,399 @@
+// fft_pencil.hip -- G_K_dF (src/G_K_dF.f:11-87) for grid edges with a
+// compile-time radix plan (16, 32, 48, 64, 96, 128, 256).
+//
+// Five passes over two half-spectrum buffers in HBM, each pass reading and
+// writing contiguous runs (whole lines or >= 64-byte tiles):
+//   A  z forward:  K4:F contraction (ddot42n order, G_K_dF.f:241-268) and a
+//                  real-to-complex FFT along z.  Two real pencils y, y+1 share
+//                  one complex FFT (a + i b) and are separated afterwards.
+//                  F[c][x][y][z] -> T1[c][kz][x][y]
+//   B  y forward:  complex FFT along y, transposing      T1 -> T2[c][kz][y][x]
+//   C  x:          FFT along x of all nine components, Green projection at
+//                  each frequency, inverse FFT along x, in place in T2
+//   B' y inverse:  inverse FFT along y, transposing       T2 -> T1
+//   D  z inverse:  complex-to-real FFT along z (two pencils per complex
+//                  FFT), scale 1/N^3                      T1 -> out[c][x][y][z]
+// In LDS every stage is an in-place Stockham radix pass: each thread loads
+// its butterfly inputs into registers, the block synchronises, and the
+// outputs are written back; radix 8/4/2 butterflies are explicit, 3/5/7 use
+// the twiddle table.  Lines are padded by one element in eight so that the
+// strided butterfly writes fall on distinct LDS banks.
+#include "kernels.h"
+
+namespace cpfft {
+namespace fp {
+
+constexpr int kT = 256;  // threads per block
+
+struct Plan {
+  int nst;
+  int r[12];
+};
+
+constexpr Plan plan_for(int n) {
+  Plan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n;
+  while (m % 8 == 0) { p.r[p.nst++] = 8; m /= 8; }
+  while (m % 4 == 0) { p.r[p.nst++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.r[p.nst++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.r[p.nst++] = 3; m /= 3; }
+  while (m % 5 == 0) { p.r[p.nst++] = 5; m /= 5; }
+  while (m % 7 == 0) { p.r[p.nst++] = 7; m /= 7; }
+  if (m != 1) p.nst = -1;
+  return p;
+}
+
+// padded position of element n of a line, and the padded line stride
+__device__ __forceinline__ constexpr int pidx(int n) { return n + (n >> 3); }
+constexpr int lstride(int n) { return n + ((n - 1) >> 3); }
+
+// pencils per block of the z passes: the largest even divisor TY of N with
+// TY*N <= 1024 (at least 2)
+constexpr int tile_z(int n) {
+  int best = 2;
+  for (int t = 2; t <= n; t += 2)
+    if (n % t == 0 && t * n <= 1024) best = t;
+  return best;
+}
+// lines per block of the y passes: the largest divisor of N up to 8
+constexpr int tile_y(int n) {
+  int best = 1;
+  for (int t = 1; t <= 8; ++t)
+    if (n % t == 0) best = t;
+  return best;
+}
+
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool FWD>
+__device__ __forceinline__ cplx mul_mi(cplx a) {
+  return FWD ? cplx{a.im, -a.re} : cplx{-a.im, a.re};
+}
+// table holds exp(-2 pi i e/N); the inverse transform uses its conjugate
+template <bool FWD>
+__device__ __forceinline__ cplx twid(const cplx* tw, int e) {
+  cplx w = tw[e];
+  if (!FWD) w.im = -w.im;
+  return w;
+}
+
+template <bool FWD>
+__device__ __forceinline__ void dft4(cplx& a0, cplx& a1, cplx& a2, cplx& a3) {
+  cplx b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3), b3 = mul_mi<FWD>(csub(a1, a3));
+  a0 = cadd(b0, b2);
+  a1 = cadd(b1, b3);
+  a2 = csub(b0, b2);
+  a3 = csub(b1, b3);
+}
+
+// in-register DFT of size R on x[0..R)
+template <int R, bool FWD, int N>
+__device__ __forceinline__ void butterfly(cplx* x, const cplx* tw) {
+  if constexpr (R == 2) {
+    cplx a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    dft4<FWD>(x[0], x[1], x[2], x[3]);
+  } else if constexpr (R == 8) {
+    // X_q = E_q + w8^q O_q, X_{q+4} = E_q - w8^q O_q with E, O the 4-point
+    // DFTs of the even and odd inputs
+    cplx e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    cplx o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<FWD>(e0, e1, e2, e3);
+    dft4<FWD>(o0, o1, o2, o3);
+    const double h = 0.70710678118654752440;  // sqrt(2)/2
+    cplx w1 = FWD ? cplx{h * (o1.re + o1.im), h * (o1.im - o1.re)}
+                  : cplx{h * (o1.re - o1.im), h * (o1.re + o1.im)};
+    cplx w2 = mul_mi<FWD>(o2);
+    cplx w3 = FWD ? cplx{h * (o3.im - o3.re), -h * (o3.re + o3.im)}
+                  : cplx{-h * (o3.re + o3.im), h * (o3.re - o3.im)};
+    x[0] = cadd(e0, o0);
+    x[4] = csub(e0, o0);
+    x[1] = cadd(e1, w1);
+    x[5] = csub(e1, w1);
+    x[2] = cadd(e2, w2);
+    x[6] = csub(e2, w2);
+    x[3] = cadd(e3, w3);
+    x[7] = csub(e3, w3);
+  } else {
+    // direct DFT with the R-th roots of unity from the N-th root table
+    cplx y[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      cplx acc = x[0];
+#pragma unroll
+      for (int t = 1; t < R; ++t) acc = cadd(acc, cmul(x[t], twid<FWD>(tw, ((q * t) % R) * (N / R))));
+      y[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) x[q] = y[q];
+  }
+}
+
+// One in-place Stockham pass of radix R over L lines of length N in LDS
+// (line l, element n at buf[l*LS + pidx(n)]); NS is the product of the
+// radices of the earlier passes.  Starts and ends with all threads' LDS
+// accesses complete.
+template <int N, int R, int NS, int L, bool FWD>
+__device__ __forceinline__ void stage(cplx* buf, const cplx* tw) {
+  constexpr int NB = N / R;             // butterflies per line
+  constexpr int TOT = L * NB;
+  constexpr int BPT = (TOT + kT - 1) / kT;
+  constexpr int TS = N / (NS * R);      // twiddle exponent step
+  constexpr int LS = lstride(N);
+  cplx x[BPT][R];
+#pragma unroll
+  for (int u = 0; u < BPT; ++u) {
+    const int b = threadIdx.x + u * kT;
+    if (TOT % kT == 0 || b < TOT) {
+      const int line = b / NB, j = b - line * NB;
+      const cplx* s = buf + line * LS;
+#pragma unroll
+      for (int t = 0; t < R; ++t) x[u][t] = s[pidx(j + t * NB)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < BPT; ++u) {
+    const int b = threadIdx.x + u * kT;
+    if (TOT % kT == 0 || b < TOT) {
+      const int line = b / NB, j = b - line * NB;
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) x[u][t] = cmul(x[u][t], twid<FWD>(tw, t * k * TS));
+      }
+      butterfly<R, FWD, N>(x[u], tw);
+      const int o = (j / NS) * NS * R + k;
+      cplx* d = buf + line * LS;
+#pragma unroll
+      for (int q = 0; q < R; ++q) d[pidx(o + q * NS)] = x[u][q];
+    }
+  }
+  __syncthreads();
+}
+
+// unscaled FFT (forward: exp(-2 pi i nk/N)) of L lines in place
+template <int N, int L, bool FWD, int ST = 0, int NS = 1>
+__device__ __forceinline__ void fft(cplx* buf, const cplx* tw) {
+  constexpr Plan P = plan_for(N);
+  static_assert(P.nst > 0, "grid edge without a compile-time radix plan");
+  if constexpr (ST < P.nst) {
+    constexpr int R = P.r[ST];
+    stage<N, R, NS, L, FWD>(buf, tw);
+    fft<N, L, FWD, ST + 1, NS * R>(buf, tw);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void load_tw(cplx* tw, const cplx* __restrict__ twg) {
+  for (int i = threadIdx.x; i < N; i += kT) tw[i] = twg[i];
+}
+
+// ---- pass A: K4:F and real-to-complex FFT along z ------------------------
+// grid (N/TY, N): blockIdx.x = y tile, blockIdx.y = x
+template <int N, bool HASK>
+__global__ __launch_bounds__(kT) void kA_zfwd(const double* __restrict__ F,
+                                              const double* __restrict__ K4,
+                                              cplx* __restrict__ T1,
+                                              const cplx* __restrict__ twg) {
+  constexpr int TY = tile_z(N), P = TY * N, PT = (P + kT - 1) / kT, LP = TY / 2;
+  constexpr int LS = lstride(N), NZ = N / 2 + 1;
+  constexpr long V = (long)N * N * N;
+  __shared__ cplx tw[N];
+  __shared__ cplx buf[LP * LS];
+  const int y0 = blockIdx.x * TY, x = blockIdx.y;
+  load_tw<N>(tw, twg);
+  const long e0 = ((long)x * N + y0) * N;  // the tile is P consecutive voxels
+  double f[HASK ? PT : 1][9];
+  if constexpr (HASK) {
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int p = threadIdx.x + u * kT;
+      if (P % kT == 0 || p < P) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d) f[u][d] = F[d * V + e0 + p];
+      }
+    }
+  }
+  double* bd = reinterpret_cast<double*>(buf);
+  for (int c = 0; c < 9; ++c) {
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int p = threadIdx.x + u * kT;
+      if (P % kT == 0 || p < P) {
+        double v;
+        if constexpr (HASK) {
+          double t[9];
+#pragma unroll
+          for (int d = 0; d < 9; ++d) t[d] = K4[(9 * c + d) * V + e0 + p] * f[u][d];
+          v = t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
+        } else {
+          v = F[c * V + e0 + p];
+        }
+        const int yy = p / N, z = p - yy * N;
+        bd[2 * ((yy >> 1) * LS + pidx(z)) + (yy & 1)] = v;
+      }
+    }
+    __syncthreads();
+    fft<N, LP, true>(buf, tw);
+    // Z = FFT(a + i b): A_k = (Z_k + conj Z_{N-k})/2, B_k = -i (Z_k - conj Z_{N-k})/2
+    for (int q = threadIdx.x; q < LP * NZ; q += kT) {
+      const int k = q / LP, m = q - k * LP;
+      const cplx* line = buf + m * LS;
+      const cplx zk = line[pidx(k)], zn = line[pidx(k == 0 ? 0 : N - k)];
+      cplx* o = T1 + (((long)c * NZ + k) * N + x) * N + y0 + 2 * m;
+      o[0] = cplx{0.5 * (zk.re + zn.re), 0.5 * (zk.im - zn.im)};
+      o[1] = cplx{0.5 * (zk.im + zn.im), -0.5 * (zk.re - zn.re)};
+    }
+    __syncthreads();
+  }
+}
+
+// ---- passes B, B': FFT along y with a transpose ---------------------------
+// FWD: T1[c][kz][x][y] -> T2[c][kz][y][x]; inverse: T2 -> T1.
+// grid (N/TX, NZ, 9)
+template <int N, bool FWD>
+__global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* __restrict__ dst,
+                                           const cplx* __restrict__ twg) {
+  constexpr int TX = tile_y(N), LS = lstride(N), NZ = N / 2 + 1;
+  __shared__ cplx tw[N];
+  __shared__ cplx buf[TX * LS];
+  const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
+  const long plane = ((long)c * NZ + kz) * N * N;
+  load_tw<N>(tw, twg);
+  if constexpr (FWD) {
+    for (int i = threadIdx.x; i < TX * N; i += kT) {
+      const int t = i / N, y = i - t * N;
+      buf[t * LS + pidx(y)] = src[plane + (long)(x0 + t) * N + y];
+    }
+  } else {
+    for (int i = threadIdx.x; i < TX * N; i += kT) {
+      const int y = i / TX, t = i - y * TX;
+      buf[t * LS + pidx(y)] = src[plane + (long)y * N + x0 + t];
+    }
+  }
+  __syncthreads();
+  fft<N, TX, FWD>(buf, tw);
+  if constexpr (FWD) {
+    for (int i = threadIdx.x; i < TX * N; i += kT) {
+      const int y = i / TX, t = i - y * TX;
+      dst[plane + (long)y * N + x0 + t] = buf[t * LS + pidx(y)];
+    }
+  } else {
+    for (int i = threadIdx.x; i < TX * N; i += kT) {
+      const int t = i / N, y = i - t * N;
+      dst[plane + (long)(x0 + t) * N + y] = buf[t * LS + pidx(y)];
+    }
+  }
+}
+
+// ---- pass C: FFT along x, Green projection, inverse FFT along x ------------
+// grid (N, NZ): blockIdx.x = y, blockIdx.y = kz
+template <int N>
+__global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __restrict__ twg,
+                                           int gamma_mode) {
+  constexpr int LS = lstride(N), NZ = N / 2 + 1;
+  __shared__ cplx tw[N];
+  __shared__ cplx buf[9 * LS];
+  const int y = blockIdx.x, kz = blockIdx.y;
+  load_tw<N>(tw, twg);
+  for (int i = threadIdx.x; i < 9 * N; i += kT) {
+    const int c = i / N, x = i - c * N;
+    buf[c * LS + pidx(x)] = T2[(((long)c * NZ + kz) * N + y) * N + x];
+  }
+  __syncthreads();
+  fft<N, 9, true>(buf, tw);
+  for (int x = threadIdx.x; x < N; x += kT) {
+    cplx A[9], out[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A[c] = buf[c * LS + pidx(x)];
+    project9(N, gamma_mode, x, y, kz, A, out);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) buf[c * LS + pidx(x)] = out[c];
+  }
+  __syncthreads();
+  fft<N, 9, false>(buf, tw);
+  for (int i = threadIdx.x; i < 9 * N; i += kT) {
+    const int c = i / N, x = i - c * N;
+    T2[(((long)c * NZ + kz) * N + y) * N + x] = buf[c * LS + pidx(x)];
+  }
+}
+
+// ---- pass D: complex-to-real FFT along z, two pencils per FFT --------------
+// grid (N/TY, N, 9): blockIdx.x = y tile, blockIdx.y = x, blockIdx.z = c
+template <int N>
+__global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, double* __restrict__ out,
+                                              const cplx* __restrict__ twg) {
+  constexpr int TY = tile_z(N), P = TY * N, LP = TY / 2, LS = lstride(N), NZ = N / 2 + 1;
+  constexpr long V = (long)N * N * N;
+  __shared__ cplx tw[N];
+  __shared__ cplx buf[LP * LS];
+  const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  load_tw<N>(tw, twg);
+  // full line of pencil pair m: Z_k = A_k + i B_k, with A_{N-k} = conj A_k
+  for (int q = threadIdx.x; q < LP * NZ; q += kT) {
+    const int k = q / LP, m = q - k * LP;
+    const cplx* s = T1 + (((long)c * NZ + k) * N + x) * N + y0 + 2 * m;
+    const cplx A = s[0], B = s[1];
+    cplx* line = buf + m * LS;
+    line[pidx(k)] = cplx{A.re - B.im, A.im + B.re};
+    if (k >= 1 && N - k >= NZ) line[pidx(N - k)] = cplx{A.re + B.im, B.re - A.im};
+  }
+  __syncthreads();
+  fft<N, LP, false>(buf, tw);
+  const double sc = 1.0 / (double)V;
+  const long e0 = ((long)x * N + y0) * N;
+  for (int p = threadIdx.x; p < P; p += kT) {
+    const int yy = p / N, z = p - yy * N;
+    const cplx v = buf[(yy >> 1) * LS + pidx(z)];
+    out[c * V + e0 + p] = ((yy & 1) ? v.im : v.re) * sc;
+  }
+}
+
+template <int N>
+void launch(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* T1, cplx* T2,
+            const cplx* tw, hipStream_t s) {
+  constexpr int TY = tile_z(N), TX = tile_y(N), NZ = N / 2 + 1;
+  static_assert(N % TY == 0 && TY % 2 == 0 && N % TX == 0, "tile shapes");
+  if (K4)
+    kA_zfwd<N, true><<<dim3(N / TY, N), kT, 0, s>>>(F, K4, T1, tw);
+  else
+    kA_zfwd<N, false><<<dim3(N / TY, N), kT, 0, s>>>(F, nullptr, T1, tw);
+  kB_y<N, true><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T1, T2, tw);
+  kC_x<N><<<dim3(N, NZ), kT, 0, s>>>(T2, tw, g.gamma_mode);
+  kB_y<N, false><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T2, T1, tw);
+  kD_zinv<N><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw);
+}
+
+}  // namespace fp
+
+bool gkdf_pencil_supported(int N) {
+  switch (N) {
+    case 16: case 32: case 48: case 64: case 96: case 128: case 256: return true;
+    default: return false;
+  }
+}
+
+bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, double* out,
+                        cplx* spec, const cplx* tw, hipStream_t s) {
+  cplx* T1 = spec;
+  cplx* T2 = spec + 9 * g.S1;
+  switch (g.N) {
+    case 16: fp::launch<16>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 32: fp::launch<32>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 48: fp::launch<48>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 64: fp::launch<64>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 96: fp::launch<96>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 128: fp::launch<128>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 256: fp::launch<256>(g, F, K4, out, T1, T2, tw, s); return true;
+    default: return false;
+  }
+}
+
+}  // namespace cpfft

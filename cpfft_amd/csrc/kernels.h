@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "cpfft_common.h"
+#include "green.h"
 #include "mm10_core.h"
 
 namespace cpfft {
@@ -19,11 +20,15 @@ struct SpecGeom {
   int tz;           // kz columns per block in the x pass
 };
 
-// Gamma:(K4:F) or Gamma:F.  spec: 9*S1 complex scratch.  tw: N twiddles.
-// If dot_with != nullptr, also accumulates sum(out * dot_with) partials into
-// partials (one per z-line block) for a fused inner product.
+// Gamma:(K4:F) or Gamma:F (K4 == nullptr).  spec: 2*9*S1 complex scratch.
+// tw: the N twiddles exp(-2 pi i k/N).  Uses the compile-time-planned path
+// of fft_pencil.hip where it exists for N, else the generic runtime plan.
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
                  const cplx* tw, hipStream_t s);
+bool gkdf_pencil_supported(int N);
+// returns false (and launches nothing) when N has no compile-time plan
+bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, double* out,
+                        cplx* spec, const cplx* tw, hipStream_t s);
 
 // reductions: result -> d_out[0..nres) on device (two-stage, fixed order)
 void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,

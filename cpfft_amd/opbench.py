@@ -1,0 +1,87 @@
+"""Operator micro-benchmark: device time of each hot operator of the solver
+step on one RVE, from HIP events recorded on the solver's stream.
+
+    python -m cpfft_amd.opbench [--workload c3_128] [--reps 40]
+
+Operators (the reference routines they replace):
+  G_K_dF with K4     G_K_dF.f:11-19, flgK = 1 (the CG matrix-vector product)
+  G_K_dF without K4  G_K_dF.f:11-19, flgK = 0 (residual, tangent_homo)
+  drive_eps_sig      drive_eps_sig.f:16-45 (Newton iteration 1 of step 1)
+  dcopy              a plain copy of one [9][V] field, the bandwidth yardstick
+                     for hardware-counter calibration
+
+Prints one JSON line with the average ms and the algorithmic HBM GB/s of each
+(bytes per voxel: 792 with K4, 144 without, 144 for the copy).
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import B, DFM, FN1, WORK, Solver
+from .configs import WORKLOADS
+
+BYTES_GKDF_K4 = 792.0   # read dF 72 + K4 648, write 72
+BYTES_GKDF = 144.0      # read 72, write 72
+BYTES_COPY = 144.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c3_128")
+    ap.add_argument("--reps", type=int, default=40)
+    ap.add_argument("--skip-material", action="store_true")
+    args = ap.parse_args()
+    wl = WORKLOADS[args.workload]
+    V = wl.N ** 3
+    s = Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    print(f"[opbench] {wl.name} N={wl.N} reps={args.reps}", flush=True)
+    # K4 of the elastic state; a smooth strain field as the operand
+    s.drive_eps_sig(1, 0)
+    rng = np.random.default_rng(1)
+    s.upload(DFM, 1e-4 * rng.standard_normal((9, V)))
+    for _ in range(2):  # warm-up, not timed
+        s.gkdf(DFM, B, 1)
+        s.gkdf(DFM, B, 0)
+        s.dcopy(DFM, WORK)
+    s.sync()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        s.gkdf(DFM, B, 1)
+    for _ in range(args.reps):
+        s.gkdf(DFM, B, 0)
+    s.sync()
+    wall = time.perf_counter() - t0
+    avg, n = s.kernel_times()
+    gk, gk0 = avg[0], avg[3]
+    # copy timing: host wall clock around a synchronised batch
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        s.dcopy(DFM, WORK)
+    s.sync()
+    cp = (time.perf_counter() - t0) / args.reps * 1e3
+    res = {
+        "workload": wl.name, "N": wl.N, "reps": args.reps,
+        "gkdf_k4_ms": gk, "gkdf_k4_GBs": BYTES_GKDF_K4 * V / gk * 1e-6,
+        "gkdf_ms": gk0, "gkdf_GBs": BYTES_GKDF * V / gk0 * 1e-6,
+        "dcopy_ms_wall": cp, "dcopy_GBs_wall": BYTES_COPY * V / cp * 1e-6,
+        "gkdf_wall_s": wall,
+    }
+    if not args.skip_material:
+        s.upload(FN1, np.tile(np.array([1.0005, 0, 0, 0, 0.99985, 0, 0, 0, 0.99985])[:, None],
+                              (1, V)))
+        s.set_timing(True)
+        for _ in range(3):
+            s.drive_eps_sig(1, 1)
+        s.sync()
+        avg, n = s.kernel_times()
+        res["drive_eps_sig_ms"] = avg[1]
+        res["voxel_updates_per_s"] = V / (avg[1] * 1e-3)
+    s.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

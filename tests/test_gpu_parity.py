@@ -31,9 +31,11 @@ def solver(N, mode, ngr=4, seed=7, mat=None, dt=0.5):
     return C.Solver(N, ang, mat or C.Material(), dt=dt, gamma_mode=mode), ang
 
 
-@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32])
+@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32, 48, 64])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_gkdf(N, mode):
+    """Generic runtime-plan path (N = 4..12) and compile-time-planned path
+    (16, 32, 48, 64) against the oracle's direct DFT."""
     rng = np.random.default_rng(10 * N + mode)
     s, _ = solver(N, mode)
     fo = O.FFT(N, mode)
@@ -123,7 +125,7 @@ def test_drive_eps_sig(jac_exact):
     s.close()
 
 
-@pytest.mark.parametrize("N,mode", [(7, 0), (8, 1)])
+@pytest.mark.parametrize("N,mode", [(7, 0), (8, 1), (16, 1)])
 def test_pcg_and_tangent_homo(N, mode):
     V = N ** 3
     s, ang = solver(N, mode)
@@ -147,7 +149,7 @@ def test_pcg_and_tangent_homo(N, mode):
     s.close()
 
 
-@pytest.mark.parametrize("N,mode,nstep", [(7, 0, 3), (8, 1, 3)])
+@pytest.mark.parametrize("N,mode,nstep", [(7, 0, 3), (8, 1, 3), (16, 1, 2)])
 def test_load_steps(N, mode, nstep):
     """The Fortran driver over the C-ABI against the oracle's FFT_nr3 loop."""
     wl = C.Workload("t", N, 4, 20261202 + 99, nstep, 0.0005 * nstep, gamma_mode=mode)
@@ -161,4 +163,54 @@ def test_load_steps(N, mode, nstep):
     assert rc == 0 and rco == 0
     assert np.array_equal(cnt[:, [0, 2, 3]], cnto[:, [0, 2, 3]])
     assert rel(Pbar, Pbo) < 1e-9
+    s.close()
+
+
+def _gradient_field(N, seed):
+    """F_ij = d u_i / d x_j of a random periodic displacement u whose Fourier
+    modes all lie strictly inside the Nyquist band (numpy's FFT, layout
+    [9][N^3] with x slowest)."""
+    rng = np.random.default_rng(seed)
+    f = np.fft.fftfreq(N, 1.0 / N)
+    fz = np.fft.rfftfreq(N, 1.0 / N)
+    band = (abs(f)[:, None, None] < N / 2) & (abs(f)[None, :, None] < N / 2) & \
+        (abs(fz)[None, None, :] < N / 2)
+    q = [f[:, None, None], f[None, :, None], fz[None, None, :]]
+    F = np.empty((9, N ** 3))
+    for i in range(3):
+        uh = (rng.standard_normal((N, N, N // 2 + 1)) +
+              1j * rng.standard_normal((N, N, N // 2 + 1))) * band
+        for j in range(3):
+            F[3 * i + j] = np.fft.irfftn(2j * np.pi / N * q[j] * uh, s=(N, N, N), axes=(0, 1, 2)).ravel()
+    return F
+
+
+@pytest.mark.parametrize("N", [96, 128])
+def test_gkdf_projection_properties_full_size(N):
+    """At the benchmark grid sizes, with no oracle: the textbook Gamma is a
+    projection (Gamma:Gamma:F = Gamma:F), it returns a compatible
+    (gradient) field unchanged, it removes the mean, and K4 = identity
+    reproduces Gamma:F."""
+    V = N ** 3
+    s, _ = solver(N, 1, ngr=8)
+    rng = np.random.default_rng(N)
+    F = rng.standard_normal((9, V))
+    s.upload(C.DFM, F)
+    s.gkdf(C.DFM, C.B, 0)
+    G1 = s.download(C.B)
+    assert abs(G1.mean(axis=1)).max() < 1e-13 * abs(G1).max()
+    s.gkdf(C.B, C.DFM, 0)
+    assert rel(s.download(C.DFM), G1) < 1e-12
+    K4 = np.zeros((81, V))
+    for i in range(9):
+        K4[10 * i] = 1.0
+    s.upload(C.K4, K4)
+    s.upload(C.DFM, F)
+    s.gkdf(C.DFM, C.B, 1)
+    assert rel(s.download(C.B), G1) < 1e-13
+    del K4
+    Fg = _gradient_field(N, N + 1)
+    s.upload(C.DFM, Fg)
+    s.gkdf(C.DFM, C.B, 0)
+    assert rel(s.download(C.B), Fg) < 1e-11
     s.close()
