@@ -89,7 +89,7 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3_128")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -60,13 +60,25 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
+def build_hostcheck():
+    """Test-only host build of the per-voxel update (tests/hostcheck), so the
+    CPU test suite can check the kernel's algebra against the oracle."""
+    src = os.path.join(ROOT, "tests", "hostcheck", "voxel_host.cpp")
+    out = os.path.join(ROOT, "tests", "hostcheck", "libvoxel_host.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+          "-Wno-unknown-pragmas", src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+
+
 def build_all():
     build_hip()
     build_fortran()
     build_oracle()
+    build_hostcheck()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "fortran": build_fortran, "oracle": build_oracle}[w]()
+        {"all": build_all, "hip": build_hip, "fortran": build_fortran, "oracle": build_oracle,
+         "hostcheck": build_hostcheck}[w]()

@@ -463,23 +463,19 @@ __global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
                                                       const double* __restrict__ sn,
                                                       double* __restrict__ snp1,
                                                       double* __restrict__ Pn1,
-                                                      double* __restrict__ K4,
+                                                      double* __restrict__ tan6,
                                                       long long* __restrict__ counters) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   Counters cnt{0, 0, 0, 0, 0};
   if (e < V) {
-    VoxelIO io;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        io.Fn[i][j] = Fn[(3 * i + j) * V + e];
-        io.Fn1[i][j] = Fn1[(3 * i + j) * V + e];
-      }
     const Orient<NS>& o = orients[oid[e]];
     voxel_update<NS>(
-        P, o, step, iter, io, [&](int w) { return sn[w * V + e]; },
+        P, o, step, iter,
+        [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
+        [&](int w) { return sn[w * V + e]; },
         [&](int w, double v) { snp1[w * V + e] = v; },
         [&](int c, double v) { Pn1[c * V + e] = v; },
-        [&](int c, double v) { K4[c * V + e] = v; }, cnt);
+        [&](int c, double v) { tan6[c * V + e] = v; }, cnt);
   }
   // wavefront-level sums of the counters, one integer atomic per wavefront
   long long c[5] = {cnt.pred_it, cnt.upd_it, cnt.cuts, cnt.failed, cnt.ls_evals};
@@ -490,22 +486,39 @@ __global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
   }
 }
 
+// K4 = dP/dF from the tangent of k_drive_eps_sig (cep2A)
+__global__ __launch_bounds__(64) void k_cep2A(long V, const double* __restrict__ Fn,
+                                              const double* __restrict__ Fn1,
+                                              const double* __restrict__ snp1,
+                                              const double* __restrict__ tan6,
+                                              double* __restrict__ K4) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= V) return;
+  double t6[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) t6[k] = snp1[k * V + e];
+  voxel_dPdF([&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; }, t6,
+             [&](int c) { return tan6[c * V + e]; },
+             [&](int c, double v) { K4[c * V + e] = v; });
+}
+
 template <int NS>
 void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
                           const int32_t* orient_id, int step, int iter, const double* Fn,
                           const double* Fn1, const double* sn, double* snp1, double* Pn1,
-                          double* K4, long long* counters, hipStream_t s) {
+                          double* tan6, double* K4, long long* counters, hipStream_t s) {
   const int bs = 64;
   unsigned g = (unsigned)((V + bs - 1) / bs);
   k_drive_eps_sig<NS><<<g, bs, 0, s>>>(V, P, orients, orient_id, step, iter, Fn, Fn1, sn, snp1,
-                                       Pn1, K4, counters);
+                                       Pn1, tan6, counters);
+  k_cep2A<<<g, bs, 0, s>>>(V, Fn, Fn1, snp1, tan6, K4);
 }
 
 template void launch_drive_eps_sig<12>(long, const MatParams&, const Orient<12>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, long long*, hipStream_t);
+                                       double*, double*, double*, double*, long long*, hipStream_t);
 template void launch_drive_eps_sig<24>(long, const MatParams&, const Orient<24>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, long long*, hipStream_t);
+                                       double*, double*, double*, double*, long long*, hipStream_t);
 
 }  // namespace cpfft

@@ -50,12 +50,13 @@ void launch_gather_k4_col(long V, int i, const double* K4, double* C, hipStream_
 void launch_k4_dot9(long V, const double* K4, const double* C, double* partials, double* d_out,
                     hipStream_t s);
 
-// per-voxel update
+// per-voxel update: stress update (state n+1, P, material tangent into
+// tan6 [36][V]) followed by its pull-back to K4 = dP/dF
 template <int NS>
 void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
                           const int32_t* orient_id, int step, int iter, const double* Fn,
                           const double* Fn1, const double* sn, double* snp1, double* Pn1,
-                          double* K4, long long* counters, hipStream_t s);
+                          double* tan6, double* K4, long long* counters, hipStream_t s);
 
 int reduce_partials_count(long n);
 

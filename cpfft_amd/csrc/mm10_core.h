@@ -12,13 +12,17 @@
 // cep2A cep2A.f:86-284, cs2p drive_eps_sig.f:1182-1224.
 //
 // Differences in arithmetic organisation from the reference (results agree
-// to rounding): the power law is evaluated once per slip system and
-// evaluation point, by repeated squaring when the rate exponent is an
-// integer; the Jacobian used by the tangent is re-formed from the saved
-// iterate instead of being kept across the line search; dP/dF is assembled
-// one (k,l) column at a time from 3x3 blocks.  The polar decomposition is
-// evaluated with the reference's operation order and without FMA
-// contraction because its Cardano eigenvalue step amplifies rounding.
+// to rounding): every sum over slip systems is accumulated with the
+// orientation's lattice-frame Schmid tensors m_s and spin vectors q_s and
+// rotated once (sum_s g_s (E m_s) = E sum_s g_s m_s, with E the rotation by
+// Rp_n^T), so no per-voxel copy of the rotated tensors is kept; the power
+// law is evaluated once per slip system and evaluation point, by repeated
+// squaring when the rate exponent is an integer; the Jacobian used by the
+// tangent is re-formed from the saved iterate instead of being kept across
+// the line search; dP/dF is assembled one (k,l) column at a time from 3x3
+// blocks.  The polar decomposition is evaluated with the reference's
+// operation order and without FMA contraction because its Cardano
+// eigenvalue step amplifies rounding.
 #pragma once
 
 #include "cpfft_common.h"
@@ -64,38 +68,54 @@ CPFFT_HD inline double pw_nm2(const MatParams& p, double au) {
   return p.int_rate ? ipow(au, (int)p.rate_n - 2) : pow(au, p.rate_n - 2.0);
 }
 
-// LU with partial pivoting + solve, n <= 7, row-major A (destroyed)
+// Gaussian elimination with partial pivoting on [A | B] followed by back
+// substitution, N <= 7, row-major A (destroyed).  Every index is static
+// once the loops are unrolled (the row exchange of each step is done with
+// selects), so A and B stay in registers.  The arithmetic is that of an LU
+// factorisation followed by forward and back substitution.
 template <int N, int NR>
 CPFFT_HD inline void lu_solve(double (&A)[N][N], double (&B)[N][NR]) {
-  int piv[N];
+#pragma unroll
   for (int j = 0; j < N; ++j) {
     int p = j;
     double amax = fabs(A[j][j]);
+#pragma unroll
     for (int i = j + 1; i < N; ++i) {
-      double v = fabs(A[i][j]);
+      const double v = fabs(A[i][j]);
       if (v > amax) { amax = v; p = i; }
     }
-    piv[j] = p;
-    if (p != j)
-      for (int k = 0; k < N; ++k) { double t = A[j][k]; A[j][k] = A[p][k]; A[p][k] = t; }
-    double rp = 1.0 / A[j][j];
-    for (int i = j + 1; i < N; ++i) A[i][j] *= rp;
+#pragma unroll
     for (int i = j + 1; i < N; ++i) {
-      double l = A[i][j];
+      const bool sw = (p == i);
+#pragma unroll
+      for (int k = j; k < N; ++k) {
+        const double a = A[j][k], b = A[i][k];
+        A[j][k] = sw ? b : a;
+        A[i][k] = sw ? a : b;
+      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double a = B[j][r], b = B[i][r];
+        B[j][r] = sw ? b : a;
+        B[i][r] = sw ? a : b;
+      }
+    }
+    const double rp = 1.0 / A[j][j];
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      const double l = A[i][j] * rp;
+#pragma unroll
       for (int k = j + 1; k < N; ++k) A[i][k] -= l * A[j][k];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) B[i][r] -= l * B[j][r];
     }
   }
-  for (int j = 0; j < N; ++j)
-    if (piv[j] != j)
-      for (int r = 0; r < NR; ++r) { double t = B[j][r]; B[j][r] = B[piv[j]][r]; B[piv[j]][r] = t; }
+#pragma unroll
   for (int r = 0; r < NR; ++r) {
-    for (int i = 1; i < N; ++i) {
-      double s = B[i][r];
-      for (int k = 0; k < i; ++k) s -= A[i][k] * B[k][r];
-      B[i][r] = s;
-    }
+#pragma unroll
     for (int i = N - 1; i >= 0; --i) {
       double s = B[i][r];
+#pragma unroll
       for (int k = i + 1; k < N; ++k) s -= A[i][k] * B[k][r];
       B[i][r] = s / A[i][i];
     }
@@ -228,139 +248,273 @@ CPFFT_HD inline double inv33(const double a[3][3], double g[3][3]) {
 }
 
 // ---------------------------------------------------------- material
-template <int NS>
-struct Point {        // crystal_state on the path (mm10_defs, mod_crystals.f:63-80)
-  double D[6];        // unrotated strain increment
-  double dg, tinc;    // slip-rate pre-factor, time increment
-  double ms[NS][6];   // current Schmid tensors (rotated by Rp_n^T)
-  double qc[NS][3];   // current spin vectors (rotated by R Rp_n^T)
+// Constants of the local problem at one (sub)step of one voxel (the
+// crystal_state of mm10_setup, mm10_a.f:830-962).  The current Schmid
+// tensor and spin vector of slip system s are E m_s and W q_s, with m_s and
+// q_s the orientation's values (Orient::ms, Orient::qs).
+struct Point {
+  double D[6];      // unrotated strain increment
+  double dg, tinc;  // slip-rate pre-factor, time increment
+  double tn[3][3];  // Rp_n^T; E = mm10_rt2rve(tn) is applied through rotE/rotET
+  double W[3][3];   // spin rotation by R_{n+1} Rp_n^T (mm10_rt2rvw)
 };
 
-template <int NS>
+// E v with E = rt2rve(t): the Voigt components (11,22,33,12,23,13) of
+// t V t^T, V the symmetric tensor with those components
+CPFFT_HD inline void rotE(const double t[3][3], const double v[6], double o[6]) {
+  const double V[3][3] = {{v[0], v[3], v[5]}, {v[3], v[1], v[4]}, {v[5], v[4], v[2]}};
+  double A[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = t[i][0] * V[0][j] + t[i][1] * V[1][j] + t[i][2] * V[2][j];
+  auto tv = [&](int i, int j) { return A[i][0] * t[j][0] + A[i][1] * t[j][1] + A[i][2] * t[j][2]; };
+  o[0] = tv(0, 0);
+  o[1] = tv(1, 1);
+  o[2] = tv(2, 2);
+  o[3] = tv(0, 1);
+  o[4] = tv(1, 2);
+  o[5] = tv(0, 2);
+}
+// E^T s with E = rt2rve(t): X = t^T S t with S the symmetric tensor whose
+// off-diagonal entries are half the shear components of s; the result is
+// (X11, X22, X33, 2 X12, 2 X23, 2 X13)
+CPFFT_HD inline void rotET(const double t[3][3], const double s[6], double o[6]) {
+  const double S[3][3] = {{s[0], 0.5 * s[3], 0.5 * s[5]},
+                          {0.5 * s[3], s[1], 0.5 * s[4]},
+                          {0.5 * s[5], 0.5 * s[4], s[2]}};
+  double A[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = S[i][0] * t[0][j] + S[i][1] * t[1][j] + S[i][2] * t[2][j];
+  auto tv = [&](int i, int j) { return t[0][i] * A[0][j] + t[1][i] * A[1][j] + t[2][i] * A[2][j]; };
+  o[0] = tv(0, 0);
+  o[1] = tv(1, 1);
+  o[2] = tv(2, 2);
+  o[3] = 2.0 * tv(0, 1);
+  o[4] = 2.0 * tv(1, 2);
+  o[5] = 2.0 * tv(0, 2);
+}
+
 struct StateN {
   double stress[6];
   double Rp[3][3];
   double tt, tt_rate, D[6];
 };
 
-// slip-system quantities at (stress, tt): u = rs/tt, p = |u|^(n-2)
-template <int NS>
-struct SlipEval {
-  double rs[NS], slip[NS], pn2[NS];
+CPFFT_HD inline void mv6(const double A[6][6], const double v[6], double o[6]) {
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+    o[a] = A[a][0] * v[0] + A[a][1] * v[1] + A[a][2] * v[2] + A[a][3] * v[3] + A[a][4] * v[4] +
+           A[a][5] * v[5];
+}
+CPFFT_HD inline void mtv6(const double A[6][6], const double v[6], double o[6]) {
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
+    o[b] = A[0][b] * v[0] + A[1][b] * v[1] + A[2][b] * v[2] + A[3][b] * v[3] + A[4][b] * v[4] +
+           A[5][b] * v[5];
+}
+CPFFT_HD inline void mv3(const double A[3][3], const double v[3], double o[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) o[a] = A[a][0] * v[0] + A[a][1] * v[1] + A[a][2] * v[2];
+}
+// C (row-major 6x6 in memory) times v
+CPFFT_HD inline void Cv6(const double* __restrict__ C, const double v[6], double o[6]) {
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+    o[a] = C[6 * a] * v[0] + C[6 * a + 1] * v[1] + C[6 * a + 2] * v[2] + C[6 * a + 3] * v[3] +
+           C[6 * a + 4] * v[4] + C[6 * a + 5] * v[5];
+}
+
+// slip increment of one system: u = rs/tt, p = |u|^(n-2),
+// slip = dg |u|^(n-1) u (mm10_slipinc, mm10_b.f:1676-1723, Voce)
+struct Slip {
+  double u, au, p, g;
 };
-
-template <int NS>
-CPFFT_HD inline void eval_slip(const MatParams& P, const Point<NS>& pt, const double s[6],
-                               double tt, SlipEval<NS>& ev) {
-  for (int i = 0; i < NS; ++i) {
-    double rs = s[0] * pt.ms[i][0] + s[1] * pt.ms[i][1] + s[2] * pt.ms[i][2] +
-                s[3] * pt.ms[i][3] + s[4] * pt.ms[i][4] + s[5] * pt.ms[i][5];
-    double u = rs / tt;
-    double au = fabs(u);
-    double p = pw_nm2(P, au);
-    ev.rs[i] = rs;
-    ev.pn2[i] = p;
-    ev.slip[i] = pt.dg * (p * au) * u;  // dg/tt |rs/tt|^(n-1) rs
-  }
+CPFFT_HD inline Slip slip_of(const MatParams& P, double dg, double rs, double tt) {
+  Slip r;
+  r.u = rs / tt;
+  r.au = fabs(r.u);
+  r.p = pw_nm2(P, r.au);
+  r.g = dg * (r.p * r.au) * r.u;
+  return r;
 }
 
-// R1 (mm10_formR1, mm10_b.f:1065-1113) from slip increments
+// Sums over slip systems at stress s and resistance tt, in the lattice
+// frame: mb = sum_s slip_s m_s, qb = sum_s slip_s q_s, sabs = sum |slip_s|.
+// The resolved shear is rs_s = s . (E m_s) = (E^T s) . m_s.
 template <int NS>
-CPFFT_HD inline void form_R1(const double* __restrict__ C, const Point<NS>& pt,
-                             const StateN<NS>& n, const double s[6], const SlipEval<NS>& ev,
-                             double R1[6], double wp[3]) {
-  double dbar[6] = {0, 0, 0, 0, 0, 0};
-  wp[0] = wp[1] = wp[2] = 0.0;
+CPFFT_HD inline void slip_sums(const MatParams& P, const Orient<NS>& o, const Point& pt,
+                               const double s[6], double tt, double mb[6], double qb[3],
+                               double* sabs) {
+  double sE[6];
+  rotET(pt.tn, s, sE);
+  double sa = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) mb[k] = 0.0;
+  qb[0] = qb[1] = qb[2] = 0.0;
+#pragma unroll 2
   for (int i = 0; i < NS; ++i) {
-    double g = ev.slip[i];
-    for (int k = 0; k < 6; ++k) dbar[k] += g * pt.ms[i][k];
-    for (int k = 0; k < 3; ++k) wp[k] += g * pt.qc[i][k];
+    const double* __restrict__ m = o.ms[i];
+    const double* __restrict__ q = o.qs[i];
+    const double rs = sE[0] * m[0] + sE[1] * m[1] + sE[2] * m[2] + sE[3] * m[3] + sE[4] * m[4] +
+                      sE[5] * m[5];
+    const Slip sl = slip_of(P, pt.dg, rs, tt);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) mb[k] += sl.g * m[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qb[k] += sl.g * q[k];
+    sa += fabs(sl.g);
   }
-  double sw[6], w1[6];
+  *sabs = sa;
+}
+
+// R1 (mm10_formR1, mm10_b.f:1065-1113): s - s_n - C (D - dbar) + 2 sym(s W)
+// with dbar = E mb, wp = W qb
+template <int NS>
+CPFFT_HD inline void form_R1(const MatParams& P, const Orient<NS>& o, const Point& pt,
+                             const StateN& n, const double s[6], double tt, double R1[6],
+                             double* sabs) {
+  double mb[6], qb[3];
+  slip_sums<NS>(P, o, pt, s, tt, mb, qb, sabs);
+  double dbar[6], wp[3], sw[6], w1[6], c[6];
+  rotE(pt.tn, mb, dbar);
+  mv3(pt.W, qb, wp);
   symsw(s, wp, sw);
+#pragma unroll
   for (int k = 0; k < 6; ++k) w1[k] = pt.D[k] - dbar[k];
-  for (int a = 0; a < 6; ++a) {
-    double c = 0.0;
-    for (int b = 0; b < 6; ++b) c += C[a * 6 + b] * w1[b];
-    R1[a] = s[a] - n.stress[a] - c + 2.0 * sw[a];
-  }
+  Cv6(o.C, w1, c);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) R1[a] = s[a] - n.stress[a] - c[a] + 2.0 * sw[a];
 }
 
-// h of the Voce law (mm10_h_voche, mm10_b.f:1888-1916), tau_l = 0
-template <int NS>
-CPFFT_HD inline double voce_h(const MatParams& P, const StateN<NS>& n, double tt,
-                              const SlipEval<NS>& ev) {
+// tt from the Voce law (mm10_h_voche, mm10_b.f:1888-1916), tau_l = 0
+CPFFT_HD inline double voce_h(const MatParams& P, const StateN& n, double tt, double sabs) {
   double h_term = 1.0 - (tt - P.tau_y) / P.tau_v;
   double f = P.m_is_one ? h_term : pow(fabs(h_term), P.voche_m) * sgn1(h_term);
-  double s = 0.0;
-  for (int i = 0; i < NS; ++i) s += fabs(ev.slip[i]);
-  return n.tt + P.theta_0 * (f * s);
+  return n.tt + P.theta_0 * (f * sabs);
 }
 
 // full residual R (mm10_formR, mm10_b.f:1029-1063); returns |R|^2 and tt_rate
 template <int NS>
-CPFFT_HD inline double form_R(const MatParams& P, const double* __restrict__ C,
-                              const Point<NS>& pt, const StateN<NS>& n, const double x[7],
-                              double R[7], double* tt_rate) {
-  SlipEval<NS> ev;
-  eval_slip(P, pt, x, x[6], ev);
-  double wp[3];
-  form_R1(C, pt, n, x, ev, R, wp);
-  double h = voce_h(P, n, x[6], ev);
+CPFFT_HD inline double form_R(const MatParams& P, const Orient<NS>& o, const Point& pt,
+                              const StateN& n, const double x[7], double R[7], double* tt_rate) {
+  double sabs;
+  form_R1<NS>(P, o, pt, n, x, x[6], R, &sabs);
+  double h = voce_h(P, n, x[6], sabs);
   R[6] = x[6] - h;
   *tt_rate = (h - n.tt) / pt.tinc;
   double d = 0.0;
+#pragma unroll
   for (int k = 0; k < 7; ++k) d += R[k] * R[k];
   return d;
 }
 
 // Jacobian (mm10_formJ11/J12/J21/J22, mm10_b.f:177-481, 1918-1983).
 // NJ = 6: stress block only (predictor); NJ = 7: full system.
+// J11 = I + IW(wp) + sum_s dgdt_s (C E m_s + 2 sym(s, W q_s)) (E m_s)^T
+//     = I + IW(wp) + (C E M + 2 [sym(s, (W Q)_c)]_c) E^T
+// with M = sum_s dgdt_s m_s m_s^T (upper triangle kept) and
+// Q = sum_s dgdt_s q_s m_s^T.
 template <int NS, int NJ>
-CPFFT_HD inline void form_J(const MatParams& P, const double* __restrict__ C,
-                            const Point<NS>& pt, const double s[6], double tt,
-                            double (&J)[NJ][NJ]) {
-  SlipEval<NS> ev;
-  eval_slip(P, pt, s, tt, ev);
-  for (int a = 0; a < NJ; ++a)
-    for (int b = 0; b < NJ; ++b) J[a][b] = 0.0;
+CPFFT_HD inline void form_J(const MatParams& P, const Orient<NS>& o, const Point& pt,
+                            const double s[6], double tt, double (&J)[NJ][NJ]) {
   const double n_ = P.rate_n;
-  double h_term = 1.0 - (tt - P.tau_y) / P.tau_v;
-  double fh = P.m_is_one ? h_term : pow(fabs(h_term), P.voche_m) * sgn1(h_term);
-  double absh_m = P.m_is_one ? fabs(h_term) : pow(fabs(h_term), P.voche_m);
-  double wp[3] = {0, 0, 0};
+  const double h_term = 1.0 - (tt - P.tau_y) / P.tau_v;
+  const double fh = P.m_is_one ? h_term : pow(fabs(h_term), P.voche_m) * sgn1(h_term);
+  const double absh_m = P.m_is_one ? fabs(h_term) : pow(fabs(h_term), P.voche_m);
+  double sE[6];
+  rotET(pt.tn, s, sE);
+  double M[6][6], Q[3][6], mb[6], qb[3], mf[6];
   double etau = 0.0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+#pragma unroll
+    for (int b = 0; b < 6; ++b) M[a][b] = 0.0;
+    mb[a] = 0.0;
+    mf[a] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    qb[k] = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) Q[k][b] = 0.0;
+  }
+#pragma unroll 2
   for (int i = 0; i < NS; ++i) {
-    double sw[6], wv[6];
-    symsw(s, pt.qc[i], sw);
-    for (int a = 0; a < 6; ++a) {
-      double c = 2.0 * sw[a];
-      for (int b = 0; b < 6; ++b) c += C[a * 6 + b] * pt.ms[i][b];
-      wv[a] = c;
-    }
-    double au = fabs(ev.rs[i] / tt);
-    double dgdt = pt.dg * n_ / tt * (ev.pn2[i] * au);  // dg n/tt^n |rs|^(n-1)
-    for (int a = 0; a < 6; ++a) {
-      double t = dgdt * wv[a];
-      for (int b = 0; b < 6; ++b) J[a][b] += t * pt.ms[i][b];
-    }
-    for (int k = 0; k < 3; ++k) wp[k] += ev.slip[i] * pt.qc[i][k];
+    const double* __restrict__ m = o.ms[i];
+    const double* __restrict__ q = o.qs[i];
+    const double rs = sE[0] * m[0] + sE[1] * m[1] + sE[2] * m[2] + sE[3] * m[3] + sE[4] * m[4] +
+                      sE[5] * m[5];
+    const Slip sl = slip_of(P, pt.dg, rs, tt);
+    const double dgdt = pt.dg * n_ / tt * (sl.p * sl.au);  // d slip / d rs
+    double dm[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) dm[a] = dgdt * m[a];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a; b < 6; ++b) M[a][b] += dm[a] * m[b];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int b = 0; b < 6; ++b) Q[k][b] += q[k] * dm[b];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) mb[k] += sl.g * m[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qb[k] += sl.g * q[k];
     if (NJ == 7) {
-      double dgdtt = -n_ / tt * ev.slip[i];
-      for (int a = 0; a < 6; ++a) J[a][NJ - 1] += wv[a] * dgdtt;
-      // J21 = -dh/dsigma
-      double u = ev.rs[i] / tt;
-      double f21 = fh * ev.pn2[i] * u;  // |h|^m sgn(h) |u|^(n-2) u
-      for (int b = 0; b < 6; ++b) J[NJ - 1][b] -= f21 * pt.ms[i][b];
-      double as = fabs(ev.slip[i]);
-      double sgs = P.jac_exact ? 1.0 : sgn1(ev.slip[i]);
-      etau += (P.voche_m * (-1.0 / P.tau_v) * as / fabs(h_term) - as * n_ / tt * sgn1(h_term) * sgs) *
+      const double f21 = fh * sl.p * sl.u;  // |h|^m sgn(h) |u|^(n-2) u
+#pragma unroll
+      for (int k = 0; k < 6; ++k) mf[k] += f21 * m[k];
+      const double as = fabs(sl.g);
+      const double sgs = P.jac_exact ? 1.0 : sgn1(sl.g);
+      etau += (P.voche_m * (-1.0 / P.tau_v) * as / fabs(h_term) -
+               as * n_ / tt * sgn1(h_term) * sgs) *
               absh_m;
     }
   }
+  // T[:, c] = C E M[:, c] + 2 sym(s, W Q[:, c]); row a of J11 is E T[a, :]^T
+  {
+    double T[6][6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double mc[6], em[6], tc[6], wq[3], sw[6];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) mc[a] = a <= c ? M[a][c] : M[c][a];
+      rotE(pt.tn, mc, em);
+      Cv6(o.C, em, tc);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) wq[k] = pt.W[k][0] * Q[0][c] + pt.W[k][1] * Q[1][c] + pt.W[k][2] * Q[2][c];
+      symsw(s, wq, sw);
+#pragma unroll
+      for (int a = 0; a < 6; ++a) T[a][c] = tc[a] + 2.0 * sw[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      double ja[6];
+      rotE(pt.tn, T[a], ja);
+#pragma unroll
+      for (int b = 0; b < 6; ++b) J[a][b] = ja[b];
+    }
+  }
+  double wp[3];
+  mv3(pt.W, qb, wp);
   if (NJ == 7) {
-    double fac = P.theta_0 * pt.dg * n_ / tt;
-    for (int b = 0; b < 6; ++b) J[NJ - 1][b] *= fac;
+    // J12 = -n/tt sum_s slip_s (C E m_s + 2 sym(s, W q_s)) = -n/tt (C E mb + 2 sym(s, wp))
+    double emb[6], cm[6], sw[6];
+    rotE(pt.tn, mb, emb);
+    Cv6(o.C, emb, cm);
+    symsw(s, wp, sw);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) J[a][NJ - 1] = (cm[a] + 2.0 * sw[a]) * (-n_ / tt);
+    // J21 = -theta_0 dg n/tt E mf, J22 = 1 - theta_0 etau
+    double emf[6];
+    rotE(pt.tn, mf, emf);
+    const double fac = P.theta_0 * pt.dg * n_ / tt;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) J[NJ - 1][b] = -emf[b] * fac;
     J[NJ - 1][NJ - 1] = 1.0 - P.theta_0 * etau;
   }
   // IW(wp) (mm10_iw, mm10_b.f:1605-1634) and identity
@@ -382,43 +536,38 @@ CPFFT_HD inline void form_J(const MatParams& P, const double* __restrict__ C,
   J[5][2] -= wp[1];
   J[5][3] += wp[0];
   J[5][4] -= wp[2];
+#pragma unroll
   for (int a = 0; a < 6; ++a) J[a][a] += 1.0;
 }
 
-// mm10_setup (mm10_a.f:830-962) for a point with strain increment D, time tinc
-template <int NS>
-CPFFT_HD inline void setup_point(const MatParams& P, const Orient<NS>& o, const double R[3][3],
-                                 const StateN<NS>& n, const double D[6], double tinc,
-                                 Point<NS>& pt) {
+// mm10_setup (mm10_a.f:830-962) for strain increment D over time tinc
+CPFFT_HD inline void setup_point(const MatParams& P, const double R[3][3], const StateN& n,
+                                 const double D[6], double tinc, Point& pt) {
+#pragma unroll
   for (int k = 0; k < 6; ++k) pt.D[k] = D[k];
   pt.tinc = tinc;
   double t1 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
   double t2 = D[3] * D[3] + D[4] * D[4] + D[5] * D[5];
   pt.dg = P.alter_mode ? P.eps_dot_0_y * tinc : sqrt((2.0 / 3.0) * (t1 + 0.5 * t2));
-  double tn[3][3], RE[6][6], RWC[3][3], w[3][3];
+  double w[3][3];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) tn[i][j] = n.Rp[j][i];
-  rt2rve(tn, RE);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) pt.tn[i][j] = n.Rp[j][i];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) w[i][j] = R[i][0] * tn[0][j] + R[i][1] * tn[1][j] + R[i][2] * tn[2][j];
-  rt2rvw(w, RWC);
-  for (int s = 0; s < NS; ++s) {
-    for (int a = 0; a < 6; ++a) {
-      double v = 0.0;
-      for (int b = 0; b < 6; ++b) v += RE[a][b] * o.ms[s][b];
-      pt.ms[s][a] = v;
-    }
-    for (int a = 0; a < 3; ++a)
-      pt.qc[s][a] = RWC[a][0] * o.qs[s][0] + RWC[a][1] * o.qs[s][1] + RWC[a][2] * o.qs[s][2];
-  }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      w[i][j] = R[i][0] * pt.tn[0][j] + R[i][1] * pt.tn[1][j] + R[i][2] * pt.tn[2][j];
+  rt2rvw(w, pt.W);
 }
 
 // mm10_solve (mm10_a.f:2860-3295). x in/out; xprev receives the iterate at
 // which the last full Jacobian was formed. Returns fail.
 template <int NS>
-CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C,
-                                const Point<NS>& pt, const StateN<NS>& n, double dtinc,
-                                double x[7], double xprev[7], double* tt_rate, Counters& cnt) {
+CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const Point& pt,
+                                const StateN& n, double dtinc, double x[7], double xprev[7],
+                                double* tt_rate, Counters& cnt) {
   const double cls = 1.0e-4, red = 0.5;
   const int mls = 10;
   int fail = 0;
@@ -426,59 +575,72 @@ CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C
   // ---- predictor: 6x6 Newton on R1 at extrapolated tt (mm10_a.f:2975-3156)
   {
     double d1[6], d2[6];
+#pragma unroll
     for (int k = 0; k < 6; ++k) { d1[k] = pt.D[k]; d2[k] = n.D[k]; }
     double dtr = (d1[0] + d1[1] + d1[2]) / 3.0;
     d1[0] -= dtr; d1[1] -= dtr; d1[2] -= dtr;
     double t1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
     double t2 = d1[3] * d1[3] + d1[4] * d1[4] + d1[5] * d1[5];
     double nn = (t1 + t2 == 0.0) ? 0.0 : 1.0 / sqrt(t1 + t2);
+#pragma unroll
     for (int k = 0; k < 6; ++k) d1[k] = (t1 + t2 == 0.0) ? 0.0 : d1[k] * nn;
     dtr = (d2[0] + d2[1] + d2[2]) / 3.0;
     d2[0] -= dtr; d2[1] -= dtr; d2[2] -= dtr;
     t1 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
     t2 = d2[3] * d2[3] + d2[4] * d2[4] + d2[5] * d2[5];
     nn = (t1 + t2 == 0.0) ? 0.0 : 1.0 / sqrt(t1 + t2);
+#pragma unroll
     for (int k = 0; k < 6; ++k) d2[k] = (t1 + t2 == 0.0) ? 0.0 : d2[k] * nn;
     double cosang = 0.0;
+#pragma unroll
     for (int k = 0; k < 6; ++k) cosang += d1[k] * d2[k];
     cosang = fmax(cosang, 0.0);
-    double x2 = x[6] + cosang * n.tt_rate * dtinc;
-    double x1[6], R1[6], wp[3];
+    const double x2 = x[6] + cosang * n.tt_rate * dtinc;
+    double x1[6], R1[6], sabs;
+#pragma unroll
     for (int k = 0; k < 6; ++k) x1[k] = x[k];
-    SlipEval<NS> ev;
-    eval_slip(P, pt, x1, x2, ev);
-    form_R1(C, pt, n, x1, ev, R1, wp);
+    form_R1<NS>(P, o, pt, n, x1, x2, R1, &sabs);
     double dot = 0.0;
+#pragma unroll
     for (int k = 0; k < 6; ++k) dot += R1[k] * R1[k];
     double nR1 = sqrt(dot);
     inR1 = nR1;
     int it = 0, pfail = 0;
     while (nR1 > P.atol1 && nR1 / inR1 > P.rtol1) {
-      double J[6][6], A[6][6], dx[6][1];
-      form_J<NS, 6>(P, C, pt, x1, x2, J);
+      double A[6][6], dx[6][1], gJ[6];
+      form_J<NS, 6>(P, o, pt, x1, x2, A);
+      // line-search slope J^T R before A is factorised
+#pragma unroll
+      for (int b = 0; b < 6; ++b) {
+        double wv = 0.0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) wv += A[a][b] * R1[a];
+        gJ[b] = wv;
+      }
+#pragma unroll
       for (int a = 0; a < 6; ++a) {
         dx[a][0] = R1[a];
-        for (int b = 0; b < 6; ++b) A[a][b] = -J[a][b];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) A[a][b] = -A[a][b];
       }
       lu_solve<6, 1>(A, dx);
       double ls1 = 0.5 * dot, ls2 = 0.0;
-      for (int b = 0; b < 6; ++b) {
-        double wv = 0.0;
-        for (int a = 0; a < 6; ++a) wv += J[a][b] * R1[a];
-        ls2 += dx[b][0] * wv;
-      }
+#pragma unroll
+      for (int b = 0; b < 6; ++b) ls2 += dx[b][0] * gJ[b];
       ls2 *= cls;
       double alpha = 1.0;
       for (int ls = 0;; ++ls) {
         double nlsx = ls1 + ls2 * alpha, xn[6];
+#pragma unroll
         for (int k = 0; k < 6; ++k) xn[k] = x1[k] + alpha * dx[k][0];
-        eval_slip(P, pt, xn, x2, ev);
-        form_R1(C, pt, n, xn, ev, R1, wp);
+        form_R1<NS>(P, o, pt, n, xn, x2, R1, &sabs);
         cnt.ls_evals++;
         dot = 0.0;
+#pragma unroll
         for (int k = 0; k < 6; ++k) dot += R1[k] * R1[k];
         nR1 = sqrt(dot);
         if (0.5 * dot <= nlsx || ls > mls) {
+#pragma unroll
           for (int k = 0; k < 6; ++k) x1[k] = xn[k];
           break;
         }
@@ -486,6 +648,7 @@ CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C
       }
       ++it;
       bool nan = false;
+#pragma unroll
       for (int k = 0; k < 6; ++k) nan |= (x1[k] != x1[k]);
       if (it > P.miter || nan) { pfail = 1; break; }
     }
@@ -493,6 +656,7 @@ CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C
     if (pfail) {
       fail = 1;
     } else {
+#pragma unroll
       for (int k = 0; k < 6; ++k) x[k] = x1[k];
       x[6] = x2;
     }
@@ -500,35 +664,45 @@ CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C
   // ---- full Newton on [R1; R2] (mm10_a.f:3162-3293)
   {
     double R[7];
-    double dot = form_R<NS>(P, C, pt, n, x, R, tt_rate);
+    double dot = form_R<NS>(P, o, pt, n, x, R, tt_rate);
     double nR = sqrt(dot), inR = nR;
     if (inR == 0.0) inR = inR1;
     int it = 0;
+#pragma unroll
     for (int k = 0; k < 7; ++k) xprev[k] = x[k];
     while ((nR > P.atol && nR / inR > P.rtol) || it < 1) {
-      double J[7][7], A[7][7], dx[7][1];
-      form_J<NS, 7>(P, C, pt, x, x[6], J);
+      double A[7][7], dx[7][1], gJ[7];
+      form_J<NS, 7>(P, o, pt, x, x[6], A);
+#pragma unroll
       for (int k = 0; k < 7; ++k) xprev[k] = x[k];
+#pragma unroll
+      for (int b = 0; b < 7; ++b) {
+        double wv = 0.0;
+#pragma unroll
+        for (int a = 0; a < 7; ++a) wv += A[a][b] * R[a];
+        gJ[b] = wv;
+      }
+#pragma unroll
       for (int a = 0; a < 7; ++a) {
         dx[a][0] = R[a];
-        for (int b = 0; b < 7; ++b) A[a][b] = -J[a][b];
+#pragma unroll
+        for (int b = 0; b < 7; ++b) A[a][b] = -A[a][b];
       }
       lu_solve<7, 1>(A, dx);
       double ls1 = 0.5 * dot, ls2 = 0.0;
-      for (int b = 0; b < 7; ++b) {
-        double wv = 0.0;
-        for (int a = 0; a < 7; ++a) wv += J[a][b] * R[a];
-        ls2 += dx[b][0] * wv;
-      }
+#pragma unroll
+      for (int b = 0; b < 7; ++b) ls2 += dx[b][0] * gJ[b];
       ls2 *= cls;
       double alpha = 1.0;
       for (int ls = 0;; ++ls) {
         double nlsx = ls1 + ls2 * alpha, xn[7];
+#pragma unroll
         for (int k = 0; k < 7; ++k) xn[k] = x[k] + alpha * dx[k][0];
-        dot = form_R<NS>(P, C, pt, n, xn, R, tt_rate);
+        dot = form_R<NS>(P, o, pt, n, xn, R, tt_rate);
         cnt.ls_evals++;
         nR = sqrt(dot);
         if (0.5 * dot <= nlsx || ls > mls) {
+#pragma unroll
           for (int k = 0; k < 7; ++k) x[k] = xn[k];
           break;
         }
@@ -536,6 +710,7 @@ CPFFT_HD inline int local_solve(const MatParams& P, const double* __restrict__ C
       }
       ++it;
       bool nan = false;
+#pragma unroll
       for (int k = 0; k < 7; ++k) nan |= (x[k] != x[k]);
       if (it > P.miter || nan) { fail = 1; break; }
     }
@@ -561,7 +736,7 @@ CPFFT_HD inline void dPdF(const double Fn[3][3], const double Fn1[3][3], const d
     return A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
            A[0][0] * A[1][2] * A[2][1] - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
   };
-  double Rt[3][3], U[3][3], Y[3][3], tmp[3][3], sigma[3][3], Uinv[3][3], tUi[3][3], RT_[3][3];
+  double Rt[3][3], U[3][3], Y[3][3], sigma[3][3], Uinv[3][3], tUi[3][3], RT_[3][3];
   double RY[3][3], RYR[3][3], RtRF[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Rt[i][j] = R[j][i];
@@ -597,197 +772,274 @@ CPFFT_HD inline void dPdF(const double Fn[3][3], const double Fn1[3][3], const d
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) D2[i][j] = L[i][j] + L[j][i];
   const int vg[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
-  for (int k = 0; k < 3; ++k)
-    for (int l = 0; l < 3; ++l) {
-      // dR(i,m)/dF(k,l), dRh(i,m)/dF(k,l), dL(i,j)/dF(k,l)
-      double dR[3][3], dRh[3][3], dL[3][3];
-      for (int i = 0; i < 3; ++i)
+  // row l / column k of a 3x3 matrix for a run-time k, l (selects keep the
+  // matrices in registers)
+  auto colk = [](const double A[3][3], int k, double o[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[i] = k == 0 ? A[i][0] : (k == 1 ? A[i][1] : A[i][2]);
+  };
+  auto rowl = [](const double A[3][3], int l, double o[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[i] = l == 0 ? A[0][i] : (l == 1 ? A[1][i] : A[2][i]);
+  };
+  // D2 Rh does not depend on (k, l)
+  double A1[3][3];
+  mm(D2, Rh, A1);
+#pragma unroll 1
+  for (int kl = 0; kl < 9; ++kl) {
+    const int k = kl / 3, l = kl - 3 * k;
+    double RYRk[3], Yl[3], RYl[3], RYk[3], RYRhk[3], Yhl[3], RYhl[3], RYhk[3], FpFl[3], Lk[3];
+    double fk[3], fl[3];
+    colk(RYR, k, RYRk);
+    rowl(Y, l, Yl);
+    colk(RY, l, RYl);
+    rowl(RY, k, RYk);
+    colk(RYRh, k, RYRhk);
+    rowl(Yh, l, Yhl);
+    colk(RYh, l, RYhl);
+    rowl(RYh, k, RYhk);
+    rowl(FpF, l, FpFl);
+    colk(L, k, Lk);
+    colk(finv, k, fk);  // finv[.][k]
+    rowl(finv, l, fl);  // finv[l][.]
+    const double flk = fk[0] * (l == 0) + fk[1] * (l == 1) + fk[2] * (l == 2);
+    // dR(i,m)/dF(k,l), dRh(i,m)/dF(k,l), dL(i,m)/dF(k,l)
+    double dR[3][3], dRh[3][3], dL[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        dR[i][m] = dY1 * (RYRk[i] * Yl[m] - RYl[i] * RYk[m]);
+        dRh[i][m] = dYh1 * (RYRhk[i] * Yhl[m] - RYhl[i] * RYhk[m]);
+        dL[i][m] = ((i == k) ? FpFl[m] : 0.0) - Lk[i] * FpFl[m];
+      }
+    // dt(i,j)/dF(k,l): dRh^T D2 Rh + Rh^T (dL + dL^T) Rh + Rh^T D2 dRh
+    double S[3][3], A2[3][3], dd[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) S[i][j] = dL[i][j] + dL[j][i];
+    mm(S, Rh, A2);  // S Rh
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) v += dRh[m][i] * A1[m][j] + Rh[m][i] * A2[m][j];
+        double w = 0.0;
+#pragma unroll
         for (int m = 0; m < 3; ++m) {
-          dR[i][m] = dY1 * (RYR[i][k] * Y[l][m] - RY[i][l] * RY[k][m]);
-          dRh[i][m] = dYh1 * (RYRh[i][k] * Yh[l][m] - RYh[i][l] * RYh[k][m]);
-          dL[i][m] = ((i == k) ? FpF[l][m] : 0.0) - L[i][k] * FpF[l][m];
+          double dm = 0.0;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) dm += D2[m][q] * dRh[q][j];
+          w += Rh[m][i] * dm;
         }
-      // dt(i,j)/dF(k,l): dRh^T D2 Rh + Rh^T (dL + dL^T) Rh + Rh^T D2 dRh
-      double S[3][3], A1[3][3], A2[3][3], dd[3][3];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) S[i][j] = dL[i][j] + dL[j][i];
-      mm(D2, Rh, A1);   // D2 Rh
-      mm(S, Rh, A2);    // S Rh
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          double v = 0.0;
-          for (int m = 0; m < 3; ++m) v += dRh[m][i] * A1[m][j] + Rh[m][i] * A2[m][j];
-          double w = 0.0;
-          for (int m = 0; m < 3; ++m) {
-            double dm = 0.0;
-            for (int q = 0; q < 3; ++q) dm += D2[m][q] * dRh[q][j];
-            w += Rh[m][i] * dm;
-          }
-          dd[i][j] = v + w;
+        dd[i][j] = v + w;
+      }
+    // tangent contraction dt'(i,j) = sum_ab cep99(ij,ab) dd(b,a)
+    double dt[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) v += C[vg[a][b]][vg[i][j]] * dd[b][a];
+        dt[i][j] = v;
+      }
+    // dP(i,j)/dF(k,l)
+    double Rdt[3][3], RdtU[3][3];
+    mm(R, dt, Rdt);
+    mm(Rdt, Uinv, RdtU);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double v = RtRF[i][j] * (J * flk) + J * RdtU[i][j];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          v -= J * sigma[i][m] * fk[j] * fl[m];
+          v += J * dR[i][m] * tUi[m][j];
+          double s4 = 0.0;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) s4 += dR[q][m] * finv[j][q];
+          v += J * RT_[i][m] * s4;
         }
-      // tangent contraction dt'(i,j) = sum_ab cep99(ij,ab) dd(b,a)
-      double dt[3][3];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          double v = 0.0;
-          for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) v += C[vg[a][b]][vg[i][j]] * dd[b][a];
-          dt[i][j] = v;
-        }
-      // dP(i,j)/dF(k,l)
-      double Rdt[3][3], RdtU[3][3];
-      mm(R, dt, Rdt);
-      mm(Rdt, Uinv, RdtU);
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          double v = RtRF[i][j] * (J * finv[l][k]) + J * RdtU[i][j];
-          for (int m = 0; m < 3; ++m) {
-            v -= J * sigma[i][m] * finv[j][k] * finv[l][m];
-            v += J * dR[i][m] * tUi[m][j];
-            double s4 = 0.0;
-            for (int q = 0; q < 3; ++q) s4 += dR[q][m] * finv[j][q];
-            v += J * RT_[i][m] * s4;
-          }
-          store(i * 27 + j * 9 + k * 3 + l, v);
-        }
-    }
+        store(i * 27 + j * 9 + kl, v);
+      }
+  }
 }
 
 // ------------------------------------------------------- voxel update
-struct VoxelIO {
-  double Fn[3][3], Fn1[3][3];
-};
-
-// Full per-voxel update.  sn: state n words (stride sstride), writes state
-// n+1 via snp1, P and K through the store functors.
-template <int NS, class LoadS, class StoreS, class StoreP, class StoreK>
-CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int step, int iter,
-                                  const VoxelIO& io, LoadS&& ldn, StoreS&& stn1, StoreP&& stP,
-                                  StoreK&& stK, Counters& cnt) {
-  // ---- kinematics (drive_eps_sig.f:207-245)
+// Kinematics of the step (drive_eps_sig.f:207-245): R_{n+1/2} and R_{n+1}
+// (polar decompositions), F_{n+1/2}^-1, and the unrotated strain increment
+// uddt = Q(R_{n+1/2}) sym(dF F_{n+1/2}^-1) (getrm1 opt 1 + qmply1).
+CPFFT_HD inline void step_kinematics(const double Fn[3][3], const double Fn1[3][3],
+                                     double rnh[3][3], double rn1[3][3], double fnhinv[3][3],
+                                     double uddt[6]) {
   double fnh[3][3], dfn[3][3];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
-      fnh[i][j] = 0.5 * (io.Fn[i][j] + io.Fn1[i][j]);
-      dfn[i][j] = io.Fn1[i][j] - io.Fn[i][j];
+      fnh[i][j] = 0.5 * (Fn[i][j] + Fn1[i][j]);
+      dfn[i][j] = Fn1[i][j] - Fn[i][j];
     }
-  double rnh[3][3], rn1[3][3], fnhinv[3][3];
   polar_R(fnh, rnh);
-  polar_R(io.Fn1, rn1);
+  polar_R(Fn1, rn1);
   inv33(fnh, fnhinv);
-  double ddt[6], uddt[6];
+  double ddt[6];
   {
 #pragma clang fp contract(off)
-    const double(&A)[3][3] = dfn;
-    const double(&B)[3][3] = fnhinv;
-    ddt[0] = A[0][0] * B[0][0] + A[0][1] * B[1][0] + A[0][2] * B[2][0];
-    ddt[3] = A[1][0] * B[0][0] + A[1][1] * B[1][0] + A[1][2] * B[2][0] + A[0][0] * B[0][1] +
-             A[0][1] * B[1][1] + A[0][2] * B[2][1];
-    ddt[5] = A[2][0] * B[0][0] + A[2][1] * B[1][0] + A[2][2] * B[2][0] + A[0][0] * B[0][2] +
-             A[0][1] * B[1][2] + A[0][2] * B[2][2];
-    ddt[1] = A[1][0] * B[0][1] + A[1][1] * B[1][1] + A[1][2] * B[2][1];
-    ddt[4] = A[2][0] * B[0][1] + A[2][1] * B[1][1] + A[2][2] * B[2][1] + A[1][0] * B[0][2] +
-             A[1][1] * B[1][2] + A[1][2] * B[2][2];
-    ddt[2] = A[2][0] * B[0][2] + A[2][1] * B[1][2] + A[2][2] * B[2][2];
-    // getrm1 opt 1 + qmply1: uddt = Q(rnh) ddt
-    const double(&r)[3][3] = rnh;
-    double q[6][6];
-    for (int a = 0; a < 3; ++a) {
-      q[a][0] = r[0][a] * r[0][a];
-      q[a][1] = r[1][a] * r[1][a];
-      q[a][2] = r[2][a] * r[2][a];
-      q[a][3] = r[0][a] * r[1][a];
-      q[a][4] = r[2][a] * r[1][a];
-      q[a][5] = r[0][a] * r[2][a];
-    }
-    const int ia[3] = {0, 1, 0}, ib[3] = {1, 2, 2};
-    for (int z = 0; z < 3; ++z) {
-      int a = ia[z], b = ib[z];
-      q[3 + z][0] = 2.0 * r[0][a] * r[0][b];
-      q[3 + z][1] = 2.0 * r[1][a] * r[1][b];
-      q[3 + z][2] = 2.0 * r[2][a] * r[2][b];
-      q[3 + z][3] = r[0][a] * r[1][b] + r[0][b] * r[1][a];
-      q[3 + z][4] = r[1][a] * r[2][b] + r[2][a] * r[1][b];
-      q[3 + z][5] = r[0][a] * r[2][b] + r[2][a] * r[0][b];
-    }
-    // reference index pattern for rows 4..6 (polar.f:735-758)
-    q[4][1] = 2.0 * r[1][2] * r[1][1];
-    q[4][2] = 2.0 * r[2][1] * r[2][2];
-    q[4][3] = r[0][1] * r[1][2] + r[1][1] * r[0][2];
-    q[4][5] = r[0][1] * r[2][2] + r[2][1] * r[0][2];
-    q[5][3] = r[0][0] * r[1][2] + r[1][0] * r[0][2];
-    q[3][4] = r[1][0] * r[2][1] + r[2][0] * r[1][1];
-    for (int j = 0; j < 6; ++j)
-      uddt[j] = q[j][0] * ddt[0] + q[j][1] * ddt[1] + q[j][2] * ddt[2] + q[j][3] * ddt[3] +
-                q[j][4] * ddt[4] + q[j][5] * ddt[5];
+  const double(*A)[3] = dfn;
+  const double(*B)[3] = fnhinv;
+  ddt[0] = A[0][0] * B[0][0] + A[0][1] * B[1][0] + A[0][2] * B[2][0];
+  ddt[3] = A[1][0] * B[0][0] + A[1][1] * B[1][0] + A[1][2] * B[2][0] + A[0][0] * B[0][1] +
+           A[0][1] * B[1][1] + A[0][2] * B[2][1];
+  ddt[5] = A[2][0] * B[0][0] + A[2][1] * B[1][0] + A[2][2] * B[2][0] + A[0][0] * B[0][2] +
+           A[0][1] * B[1][2] + A[0][2] * B[2][2];
+  ddt[1] = A[1][0] * B[0][1] + A[1][1] * B[1][1] + A[1][2] * B[2][1];
+  ddt[4] = A[2][0] * B[0][1] + A[2][1] * B[1][1] + A[2][2] * B[2][1] + A[1][0] * B[0][2] +
+           A[1][1] * B[1][2] + A[1][2] * B[2][2];
+  ddt[2] = A[2][0] * B[0][2] + A[2][1] * B[1][2] + A[2][2] * B[2][2];
+  const double(*r)[3] = rnh;
+  double q[6][6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    q[a][0] = r[0][a] * r[0][a];
+    q[a][1] = r[1][a] * r[1][a];
+    q[a][2] = r[2][a] * r[2][a];
+    q[a][3] = r[0][a] * r[1][a];
+    q[a][4] = r[2][a] * r[1][a];
+    q[a][5] = r[0][a] * r[2][a];
+  }
+  const int ia[3] = {0, 1, 0}, ib[3] = {1, 2, 2};
+#pragma unroll
+  for (int z = 0; z < 3; ++z) {
+    const int a = ia[z], b = ib[z];
+    q[3 + z][0] = 2.0 * r[0][a] * r[0][b];
+    q[3 + z][1] = 2.0 * r[1][a] * r[1][b];
+    q[3 + z][2] = 2.0 * r[2][a] * r[2][b];
+    q[3 + z][3] = r[0][a] * r[1][b] + r[0][b] * r[1][a];
+    q[3 + z][4] = r[1][a] * r[2][b] + r[2][a] * r[1][b];
+    q[3 + z][5] = r[0][a] * r[2][b] + r[2][a] * r[0][b];
+  }
+  // reference index pattern for rows 4..6 (polar.f:735-758)
+  q[4][1] = 2.0 * r[1][2] * r[1][1];
+  q[4][2] = 2.0 * r[2][1] * r[2][2];
+  q[4][3] = r[0][1] * r[1][2] + r[1][1] * r[0][2];
+  q[4][5] = r[0][1] * r[2][2] + r[2][1] * r[0][2];
+  q[5][3] = r[0][0] * r[1][2] + r[1][0] * r[0][2];
+  q[3][4] = r[1][0] * r[2][1] + r[2][0] * r[1][1];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    uddt[j] = q[j][0] * ddt[0] + q[j][1] * ddt[1] + q[j][2] * ddt[2] + q[j][3] * ddt[3] +
+              q[j][4] * ddt[4] + q[j][5] * ddt[5];
+  }
+}
+
+// Per-voxel stress update.  ldF(0|1, 3i+j) loads F_n | F_{n+1}; ldn(w)
+// loads state-n word w; stn1(w, v) stores state n+1; stP(3i+j, v) stores
+// P; stT(6a+b, v) stores the symmetrised material tangent (row-major 6x6)
+// that voxel_dPdF pulls back to dP/dF.
+template <int NS, class LoadF, class LoadS, class StoreS, class StoreP, class StoreT>
+CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int step, int iter,
+                                  LoadF&& ldF, LoadS&& ldn, StoreS&& stn1, StoreP&& stP,
+                                  StoreT&& stT, Counters& cnt) {
+  double uddt[6], rn1[3][3];
+  {
+    double Fn[3][3], Fn1[3][3], rnh[3][3], fnhinv[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        Fn[i][j] = ldF(0, 3 * i + j);
+        Fn1[i][j] = ldF(1, 3 * i + j);
+      }
+    step_kinematics(Fn, Fn1, rnh, rn1, fnhinv, uddt);
   }
 
   // ---- state n (mm10_copy_cc_hist; re-initialised at step 1, mm10_a.f:89-95)
-  StateN<NS> n;
-  double slipsum_n[NS], work_n[3];
+  StateN n;
   if (step == 1) {
+#pragma unroll
     for (int k = 0; k < 6; ++k) { n.stress[k] = 0.0; n.D[k] = 0.0; }
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
       for (int j = 0; j < 3; ++j) n.Rp[i][j] = (i == j) ? 1.0 : 0.0;
     n.tt = P.tau_y + 1.0e-5;
     n.tt_rate = ldn(16);
-    for (int i = 0; i < NS; ++i) slipsum_n[i] = 0.0;
-    work_n[0] = work_n[1] = work_n[2] = 0.0;
   } else {
+#pragma unroll
     for (int k = 0; k < 6; ++k) n.stress[k] = ldn(k);
+#pragma unroll
     for (int j = 0; j < 3; ++j)
+#pragma unroll
       for (int i = 0; i < 3; ++i) n.Rp[i][j] = ldn(6 + 3 * j + i);
     n.tt = ldn(15);
     n.tt_rate = ldn(16);
+#pragma unroll
     for (int k = 0; k < 6; ++k) n.D[k] = ldn(17 + k);
-    for (int i = 0; i < NS; ++i) slipsum_n[i] = ldn(23 + NS + i);
-    for (int k = 0; k < 3; ++k) work_n[k] = ldn(23 + 2 * NS + k);
   }
 
   const double* __restrict__ C = o.C;
-  Point<NS> pt;
-  setup_point(P, o, rn1, n, uddt, P.dt, pt);
+  Point pt;
+  setup_point(P, rn1, n, uddt, P.dt, pt);
 
   double s1[6], tt1, ttrate1 = 0.0;
   double tangent[6][6];
   double Rp1[3][3];
-  double slip1[NS];
   double winc = 0.0, pwinc = 0.0, psinc = 0.0;
   int failed = 0;
-  for (int i = 0; i < NS; ++i) slip1[i] = 0.0;
+  bool plastic = false;  // slip increments come from a converged plastic update
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = 0; j < 3; ++j) Rp1[i][j] = 0.0;
 
   double temp1 = 0.0, temp2 = 0.0;
+#pragma unroll
   for (int k = 0; k < 6; ++k) { temp1 += uddt[k] * uddt[k]; temp2 += n.stress[k] * n.stress[k]; }
   const bool no_load = (temp2 == 0.0) && (temp1 == 0.0);
   if (iter == 0 || no_load) {
     // explicit update with the elastic tangent (mm10_a.f:2735-2753)
+#pragma unroll
     for (int a = 0; a < 6; ++a)
+#pragma unroll
       for (int b = 0; b < 6; ++b) tangent[a][b] = C[a * 6 + b];
+#pragma unroll
     for (int k = 0; k < 6; ++k) s1[k] = n.stress[k];
     if (!no_load) {
-      SlipEval<NS> ev;
-      double R1[6], wp[3];
-      eval_slip(P, pt, n.stress, n.tt, ev);
-      form_R1(C, pt, n, n.stress, ev, R1, wp);
+      double R1[6], sabs;
+      form_R1<NS>(P, o, pt, n, n.stress, n.tt, R1, &sabs);
+#pragma unroll
       for (int k = 0; k < 6; ++k) s1[k] = n.stress[k] - R1[k];
     }
     tt1 = n.tt;
   } else {
-    // adaptive sub-stepping (mm10_a.f:2759-2843)
+    // adaptive sub-stepping (mm10_a.f:2759-2843).  A successful pass ends
+    // with frac + stp == 1 exactly (dyadic fractions), so pt then holds the
+    // full-step point again.
     double x[7], xprev[7], ost[6], ott = n.tt;
+#pragma unroll
     for (int k = 0; k < 6; ++k) { x[k] = n.stress[k]; ost[k] = x[k]; }
     double frac = 0.0, stp = 1.0;
     int cuts = 0, fail = 0;
-    Point<NS> cur;
     while (frac < 1.0) {
-      double Dw[6];
-      for (int k = 0; k < 6; ++k) Dw[k] = uddt[k] * (stp + frac);
-      setup_point(P, o, rn1, n, Dw, P.dt * (stp + frac), cur);
+      if (frac != 0.0 || stp != 1.0) {
+        double Dw[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Dw[k] = uddt[k] * (stp + frac);
+        setup_point(P, rn1, n, Dw, P.dt * (stp + frac), pt);
+      }
       x[6] = n.tt;
-      fail = local_solve<NS>(P, C, cur, n, P.dt * stp, x, xprev, &ttrate1, cnt);
+      fail = local_solve<NS>(P, o, pt, n, P.dt * stp, x, xprev, &ttrate1, cnt);
       if (fail) {
+#pragma unroll
         for (int k = 0; k < 6; ++k) x[k] = ost[k];
         x[6] = ott;
         stp *= 0.5;
@@ -796,101 +1048,117 @@ CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int s
         if (cuts > 4) break;
         fail = 0;
       } else {
+#pragma unroll
         for (int k = 0; k < 6; ++k) ost[k] = x[k];
         ott = x[6];
         frac += stp;
       }
     }
     bool bad = fail || (x[6] != x[6]);
+#pragma unroll
     for (int k = 0; k < 6; ++k) bad |= (x[k] != x[k]);
     if (bad) {
       // The reference abandons the rest of its element block here
       // (mm10_a.f:113-116, 255-258); this implementation reverts the voxel
       // to state n with the elastic tangent and counts it.
       failed = 1;
+#pragma unroll
       for (int k = 0; k < 6; ++k) s1[k] = n.stress[k];
       tt1 = n.tt;
       ttrate1 = 0.0;
+#pragma unroll
       for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j) Rp1[i][j] = n.Rp[i][j];
+#pragma unroll
       for (int a = 0; a < 6; ++a)
+#pragma unroll
         for (int b = 0; b < 6; ++b) tangent[a][b] = C[a * 6 + b];
     } else {
+      plastic = true;
+#pragma unroll
       for (int k = 0; k < 6; ++k) s1[k] = x[k];
       tt1 = x[6];
-      // consistent tangent (mm10_tangent) from the Jacobian at xprev for the
-      // last sub-step point, symmetrised (mm10_a.f:1134-1142)
+      // consistent tangent (mm10_tangent) from the Jacobian at xprev,
+      // symmetrised (mm10_a.f:1134-1142)
       {
         double J[7][7];
-        form_J<NS, 7>(P, C, cur, xprev, xprev[6], J);
+        form_J<NS, 7>(P, o, pt, xprev, xprev[6], J);
         double JJ[6][6], B[6][6];
-        double j22 = J[6][6];
+        const double j22 = J[6][6];
+#pragma unroll
         for (int a = 0; a < 6; ++a)
+#pragma unroll
           for (int b = 0; b < 6; ++b) JJ[a][b] = J[a][b] - J[a][6] * (J[6][b] / j22);
+#pragma unroll
         for (int a = 0; a < 6; ++a)
+#pragma unroll
           for (int b = 0; b < 6; ++b) B[a][b] = C[a * 6 + b];
         lu_solve<6, 6>(JJ, B);
+#pragma unroll
         for (int a = 0; a < 6; ++a)
+#pragma unroll
           for (int b = 0; b < 6; ++b) tangent[a][b] = 0.5 * (B[a][b] + B[b][a]);
       }
+      // slip sums at the converged state
+      double mb[6], qb[3], sabs;
+      slip_sums<NS>(P, o, pt, s1, tt1, mb, qb, &sabs);
       // plastic rotation (mm10_update_rotation): Rp = expm(W(wbarp)) Rp_n
-      SlipEval<NS> ev;
-      eval_slip(P, pt, s1, tt1, ev);
       {
-        double RW[3][3], tn[3][3];
+        double RW[3][3], tn[3][3], wb[3];
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
           for (int j = 0; j < 3; ++j) tn[i][j] = n.Rp[j][i];
         rt2rvw(tn, RW);
-        double wb[3] = {0, 0, 0};
-        for (int s = 0; s < NS; ++s) {
-          double qs[3];
-          for (int a = 0; a < 3; ++a)
-            qs[a] = RW[a][0] * o.qs[s][0] + RW[a][1] * o.qs[s][1] + RW[a][2] * o.qs[s][2];
-          for (int a = 0; a < 3; ++a) wb[a] += ev.slip[s] * qs[a];
-        }
+        mv3(RW, qb, wb);
         double W[3][3] = {{0.0, wb[2], wb[1]}, {-wb[2], 0.0, wb[0]}, {-wb[1], -wb[0], 0.0}};
         double al = sqrt(W[1][2] * W[1][2] + W[0][2] * W[0][2] + W[0][1] * W[0][1]);
         double E[3][3];
         if (al < 1.0e-16) {
+#pragma unroll
           for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) E[i][j] = 0.0;
         } else {
           double ca = (1.0 - cos(al)) / (al * al), sa = sin(al) / al;
+#pragma unroll
           for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j)
               E[i][j] = sa * W[i][j] + ca * (W[i][0] * W[0][j] + W[i][1] * W[1][j] + W[i][2] * W[2][j]);
         }
         E[0][0] += 1.0; E[1][1] += 1.0; E[2][2] += 1.0;
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
           for (int j = 0; j < 3; ++j)
             Rp1[i][j] = E[i][0] * n.Rp[0][j] + E[i][1] * n.Rp[1][j] + E[i][2] * n.Rp[2][j];
       }
-      // outputs kept in history (mm10_output): slip increments, work terms
-      for (int s = 0; s < NS; ++s) slip1[s] = ev.slip[s];
+      // work terms kept in history (mm10_output)
       winc = s1[0] * uddt[0] + s1[1] * uddt[1] + s1[2] * uddt[2] + s1[3] * uddt[3] +
              s1[4] * uddt[4] + s1[5] * uddt[5];
       double ee[6];
       {
         double S[6][6], b[6][1];
+#pragma unroll
         for (int a = 0; a < 6; ++a) {
           b[a][0] = s1[a];
+#pragma unroll
           for (int c = 0; c < 6; ++c) S[a][c] = C[a * 6 + c];
         }
         lu_solve<6, 1>(S, b);  // C^-1 sigma (SPD solve, mm10_a.f:3537)
-        double er[6][6];
+        double er[6][6], bv[6];
         rt2rve(rn1, er);
-        for (int a = 0; a < 6; ++a) {
-          double v = 0.0;
-          for (int c = 0; c < 6; ++c) v += er[a][c] * b[c][0];
-          ee[a] = v;
-        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) bv[c] = b[c][0];
+        mv6(er, bv, ee);
       }
-      double dbar[6] = {0, 0, 0, 0, 0, 0}, wp[3] = {0, 0, 0}, ewwe[6], ep[6];
-      for (int s = 0; s < NS; ++s) {
-        for (int k = 0; k < 6; ++k) dbar[k] += slip1[s] * pt.ms[s][k];
-        for (int k = 0; k < 3; ++k) wp[k] += slip1[s] * pt.qc[s][k];
-      }
+      double dbar[6], wp[3], ewwe[6], ep[6];
+      rotE(pt.tn, mb, dbar);
+      mv3(pt.W, qb, wp);
       symsw(ee, wp, ewwe);
+#pragma unroll
       for (int k = 0; k < 6; ++k) ep[k] = dbar[k] + ewwe[k];
       double a1 = ep[0] * ep[0] + ep[1] * ep[1] + ep[2] * ep[2];
       double a2 = ep[3] * ep[3] + ep[4] * ep[4] + ep[5] * ep[5];
@@ -902,38 +1170,96 @@ CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int s
   cnt.failed += failed;
 
   // ---- store state n+1 (mm10_store_cryhist, mm10_a_store_crystal)
+#pragma unroll
   for (int k = 0; k < 6; ++k) stn1(k, s1[k]);
+#pragma unroll
   for (int j = 0; j < 3; ++j)
+#pragma unroll
     for (int i = 0; i < 3; ++i) stn1(6 + 3 * j + i, Rp1[i][j]);
   stn1(15, tt1);
   stn1(16, ttrate1);
+#pragma unroll
   for (int k = 0; k < 6; ++k) stn1(17 + k, uddt[k]);
-  for (int s = 0; s < NS; ++s) stn1(23 + s, slip1[s]);
-  for (int s = 0; s < NS; ++s) stn1(23 + NS + s, slipsum_n[s] + slip1[s]);
-  stn1(23 + 2 * NS + 0, work_n[0] + winc);
-  stn1(23 + 2 * NS + 1, work_n[1] + pwinc);
-  stn1(23 + 2 * NS + 2, work_n[2] + psinc);
-
-  // ---- Cauchy stress and P = J sigma F^-T (drive_eps_sig.f:262-273)
-  double fn1inv[3][3];
-  double detF = inv33(io.Fn1, fn1inv);
-  double cs[3][3];
   {
-    double ts[3][3] = {{s1[0], s1[3], s1[5]}, {s1[3], s1[1], s1[4]}, {s1[5], s1[4], s1[2]}};
-    double tmp[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        tmp[i][j] = rn1[i][0] * ts[0][j] + rn1[i][1] * ts[1][j] + rn1[i][2] * ts[2][j];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        cs[i][j] = tmp[i][0] * rn1[j][0] + tmp[i][1] * rn1[j][1] + tmp[i][2] * rn1[j][2];
+    // slip increments and cumulative slip (zero outside a converged plastic update)
+    double sE[6];
+    rotET(pt.tn, s1, sE);
+#pragma unroll 2
+    for (int i = 0; i < NS; ++i) {
+      double g = 0.0;
+      if (plastic) {
+        const double* __restrict__ m = o.ms[i];
+        const double rs = sE[0] * m[0] + sE[1] * m[1] + sE[2] * m[2] + sE[3] * m[3] +
+                          sE[4] * m[4] + sE[5] * m[5];
+        g = slip_of(P, pt.dg, rs, tt1).g;
+      }
+      stn1(23 + i, g);
+      stn1(23 + NS + i, (step == 1 ? 0.0 : ldn(23 + NS + i)) + g);
+    }
   }
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      stP(3 * i + j, detF * (cs[i][0] * fn1inv[j][0] + cs[i][1] * fn1inv[j][1] + cs[i][2] * fn1inv[j][2]));
+  const double w0 = step == 1 ? 0.0 : ldn(23 + 2 * NS + 0);
+  const double w1 = step == 1 ? 0.0 : ldn(23 + 2 * NS + 1);
+  const double w2 = step == 1 ? 0.0 : ldn(23 + 2 * NS + 2);
+  stn1(23 + 2 * NS + 0, w0 + winc);
+  stn1(23 + 2 * NS + 1, w1 + pwinc);
+  stn1(23 + 2 * NS + 2, w2 + psinc);
 
-  // ---- K4 = dP/dF (cep2A)
-  dPdF(io.Fn, io.Fn1, s1, tangent, rnh, fnhinv, rn1, fn1inv, detF, stK);
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) stT(6 * a + b, tangent[a][b]);
+
+  // ---- Cauchy stress sigma = R t R^T and P = J sigma F^-T (drive_eps_sig.f:262-273, cs2p)
+  double Fn1[3][3], fn1inv[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Fn1[i][j] = ldF(1, 3 * i + j);
+  const double detF = inv33(Fn1, fn1inv);
+  double cs[3][3], tmp[3][3];
+  const double ts[3][3] = {{s1[0], s1[3], s1[5]}, {s1[3], s1[1], s1[4]}, {s1[5], s1[4], s1[2]}};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      tmp[i][j] = rn1[i][0] * ts[0][j] + rn1[i][1] * ts[1][j] + rn1[i][2] * ts[2][j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      cs[i][j] = tmp[i][0] * rn1[j][0] + tmp[i][1] * rn1[j][1] + tmp[i][2] * rn1[j][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      stP(3 * i + j,
+          detF * (cs[i][0] * fn1inv[j][0] + cs[i][1] * fn1inv[j][1] + cs[i][2] * fn1inv[j][2]));
+}
+
+// K4 = dP/dF (cep2A, cep2A.f:86-284) of one voxel from F_n, F_{n+1}, the
+// unrotated Cauchy stress t (state n+1 words 0..5) and the material tangent
+// written by voxel_update; the kinematics are re-evaluated from F.
+template <class LoadF, class LoadT, class StoreK>
+CPFFT_HD inline void voxel_dPdF(LoadF&& ldF, const double t6[6], LoadT&& ldT, StoreK&& stK) {
+  double Fn[3][3], Fn1[3][3], rnh[3][3], rn1[3][3], fnhinv[3][3], fn1inv[3][3], fnh[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Fn[i][j] = ldF(0, 3 * i + j);
+      Fn1[i][j] = ldF(1, 3 * i + j);
+      fnh[i][j] = 0.5 * (Fn[i][j] + Fn1[i][j]);
+    }
+  polar_R(fnh, rnh);
+  polar_R(Fn1, rn1);
+  inv33(fnh, fnhinv);
+  const double detF = inv33(Fn1, fn1inv);
+  double tangent[6][6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) tangent[a][b] = ldT(6 * a + b);
+  dPdF(Fn, Fn1, t6, tangent, rnh, fnhinv, rn1, fn1inv, detF, stK);
 }
 
 }  // namespace cpfft

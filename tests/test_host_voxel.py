@@ -1,0 +1,101 @@
+"""CPU check of the library's per-voxel update algebra: the device source
+(cpfft_amd/csrc/mm10_core.h) compiled for the host (tests/hostcheck) against
+the oracle's drive_eps_sig on identical inputs.
+
+The device code accumulates every sum over slip systems in the lattice frame
+and rotates it once; the oracle follows the reference and rotates each slip
+system's Schmid tensor.  Same mathematics, different rounding: per-field
+agreement within 1e-9 relative (the bar of the GPU parity tests).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import cpfft_amd as C
+import oracle_lib as O
+from cpfft_amd.microstructure import polycrystal
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "hostcheck", "libvoxel_host.so")
+
+
+def host_lib():
+    if not os.path.exists(LIB):
+        pytest.skip("host check library not built (python -m cpfft_amd.build hostcheck)")
+    L = ctypes.CDLL(LIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.cpfft_host_drive_eps_sig.argtypes = [ctypes.POINTER(C.CMaterial), ctypes.c_double,
+                                           ctypes.c_long, dp, ctypes.c_int, ctypes.c_int, dp, dp,
+                                           dp, dp, dp, dp, ctypes.POINTER(ctypes.c_longlong)]
+    return L
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def host_drive(mat, dt, ang, step, it, Fn, Fn1, sn):
+    L = host_lib()
+    cm = C.CMaterial(**{k: getattr(mat, k) for k, _ in C.CMaterial._fields_})
+    V = Fn.shape[1]
+    snp1 = np.zeros_like(sn)
+    Pn1 = np.zeros((9, V))
+    K4 = np.zeros((81, V))
+    cnt = (ctypes.c_longlong * 5)()
+    L.cpfft_host_drive_eps_sig(ctypes.byref(cm), dt, V, P(np.ascontiguousarray(ang)), step, it,
+                               P(np.ascontiguousarray(Fn)), P(np.ascontiguousarray(Fn1)),
+                               P(np.ascontiguousarray(sn)), P(snp1), P(Pn1), P(K4), cnt)
+    return Pn1, K4, snp1, list(cnt)
+
+
+def rel(a, b):
+    return abs(a - b).max() / max(abs(b).max(), 1e-300)
+
+
+def near_identity(rng, V, amp, uniaxial=0.0):
+    F = np.zeros((9, V))
+    F[[0, 4, 8]] = 1.0
+    F += amp * rng.standard_normal((9, V))
+    F[0] += uniaxial
+    F[4] -= 0.5 * uniaxial
+    F[8] -= 0.5 * uniaxial
+    return F
+
+
+def check(a, b, nslip, what):
+    Pn, Kn, sn, cn = a
+    Po, Ko, so, st = b
+    assert cn[3] == st[3], what
+    assert rel(Pn, Po) < 1e-9, what
+    assert rel(Kn, Ko) < 1e-9, what
+    for w0, w1 in [(0, 6), (6, 15), (15, 17), (17, 23), (23, 23 + 2 * nslip),
+                   (23 + 2 * nslip, 26 + 2 * nslip)]:
+        assert rel(sn[w0:w1], so[w0:w1]) < 1e-9, (what, w0)
+
+
+@pytest.mark.parametrize("slip_set,jac_exact", [(1, 0), (1, 1), (8, 1)])
+def test_host_update_matches_oracle(slip_set, jac_exact):
+    V = 6 ** 3
+    rng = np.random.default_rng(40 + slip_set + jac_exact)
+    _, ang = polycrystal(6, 5, 11 + slip_set)
+    mat = C.Material(slip_set=slip_set, jac_exact=jac_exact)
+    oc = O.crystal(slip_set=slip_set, jac_exact=jac_exact)
+    dt = 0.25
+    Fn = near_identity(rng, V, 0.0)
+    Fn1 = near_identity(rng, V, 1e-4, uniaxial=6e-4)
+    sn = O.state_init(oc, V)
+    for step, it in [(1, 0), (1, 1), (1, 2)]:
+        h = host_drive(mat, dt, ang, step, it, Fn, Fn1, sn)
+        o = O.drive_eps_sig(oc, ang, step, it, dt, Fn, Fn1, sn)
+        check(h, o, oc.nslip, (step, it))
+    # a second load step from the converged state
+    sn = o[2]
+    Fn = Fn1
+    Fn1 = Fn + near_identity(rng, V, 5e-5, uniaxial=5e-4) - near_identity(rng, V, 0.0)
+    for it in (1, 2):
+        h = host_drive(mat, dt, ang, 2, it, Fn, Fn1, sn)
+        o = O.drive_eps_sig(oc, ang, 2, it, dt, Fn, Fn1, sn)
+        check(h, o, oc.nslip, (2, it))
+    assert h[3][1] > 0  # plastic Newton iterations happened
