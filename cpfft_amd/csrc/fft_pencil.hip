@@ -139,18 +139,18 @@ __device__ __forceinline__ void butterfly(cplx* x, const cplx* tw) {
 // (line l, element n at buf[l*LS + pidx(n)]); NS is the product of the
 // radices of the earlier passes.  Starts and ends with all threads' LDS
 // accesses complete.
-template <int N, int R, int NS, int L, bool FWD>
+template <int N, int R, int NS, int L, bool FWD, int T>
 __device__ __forceinline__ void stage(cplx* buf, const cplx* tw) {
   constexpr int NB = N / R;             // butterflies per line
   constexpr int TOT = L * NB;
-  constexpr int BPT = (TOT + kT - 1) / kT;
+  constexpr int BPT = (TOT + T - 1) / T;
   constexpr int TS = N / (NS * R);      // twiddle exponent step
   constexpr int LS = lstride(N);
   cplx x[BPT][R];
 #pragma unroll
   for (int u = 0; u < BPT; ++u) {
-    const int b = threadIdx.x + u * kT;
-    if (TOT % kT == 0 || b < TOT) {
+    const int b = threadIdx.x + u * T;
+    if (TOT % T == 0 || b < TOT) {
       const int line = b / NB, j = b - line * NB;
       const cplx* s = buf + line * LS;
 #pragma unroll
@@ -160,8 +160,8 @@ __device__ __forceinline__ void stage(cplx* buf, const cplx* tw) {
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < BPT; ++u) {
-    const int b = threadIdx.x + u * kT;
-    if (TOT % kT == 0 || b < TOT) {
+    const int b = threadIdx.x + u * T;
+    if (TOT % T == 0 || b < TOT) {
       const int line = b / NB, j = b - line * NB;
       const int k = j % NS;
       if constexpr (NS > 1) {
@@ -178,74 +178,101 @@ __device__ __forceinline__ void stage(cplx* buf, const cplx* tw) {
   __syncthreads();
 }
 
-// unscaled FFT (forward: exp(-2 pi i nk/N)) of L lines in place
-template <int N, int L, bool FWD, int ST = 0, int NS = 1>
+// unscaled FFT (forward: exp(-2 pi i nk/N)) of L lines in place by a block
+// of T threads
+template <int N, int L, bool FWD, int T = kT, int ST = 0, int NS = 1>
 __device__ __forceinline__ void fft(cplx* buf, const cplx* tw) {
   constexpr Plan P = plan_for(N);
   static_assert(P.nst > 0, "grid edge without a compile-time radix plan");
   if constexpr (ST < P.nst) {
     constexpr int R = P.r[ST];
-    stage<N, R, NS, L, FWD>(buf, tw);
-    fft<N, L, FWD, ST + 1, NS * R>(buf, tw);
+    stage<N, R, NS, L, FWD, T>(buf, tw);
+    fft<N, L, FWD, T, ST + 1, NS * R>(buf, tw);
   }
 }
 
-template <int N>
+template <int N, int T = kT>
 __device__ __forceinline__ void load_tw(cplx* tw, const cplx* __restrict__ twg) {
-  for (int i = threadIdx.x; i < N; i += kT) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < N; i += T) tw[i] = twg[i];
 }
 
+// threads per block of pass A: two voxels per thread with K4 (its registers
+// hold F and two K4 row blocks), four without
+constexpr int threads_z(int n, bool hask) { return hask && tile_z(n) * n / 2 >= 512 ? 512 : 256; }
+
 // ---- pass A: K4:F and real-to-complex FFT along z ------------------------
-// grid (N/TY, N): blockIdx.x = y tile, blockIdx.y = x
+// grid (N/TY, N): blockIdx.x = y tile, blockIdx.y = x.  The nine output
+// components are transformed one after the other; the K4 rows (or F
+// component) of component c+1 are loaded into a second register set before
+// component c is transformed, so the loads overlap the LDS work.
 template <int N, bool HASK>
-__global__ __launch_bounds__(kT) void kA_zfwd(const double* __restrict__ F,
-                                              const double* __restrict__ K4,
-                                              cplx* __restrict__ T1,
-                                              const cplx* __restrict__ twg,
-                                              const int* __restrict__ skip) {
+__global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* __restrict__ F,
+                                                              const double* __restrict__ K4,
+                                                              cplx* __restrict__ T1,
+                                                              const cplx* __restrict__ twg,
+                                                              const int* __restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int TY = tile_z(N), P = TY * N, PT = (P + kT - 1) / kT, LP = TY / 2;
+  constexpr int TA = threads_z(N, HASK);
+  constexpr int TY = tile_z(N), P = TY * N, PT = (P + TA - 1) / TA, LP = TY / 2;
   constexpr int LS = lstride(N), NZ = N / 2 + 1;
+  constexpr int NK = HASK ? 9 : 1;  // values per point loaded per component
   constexpr long V = (long)N * N * N;
   __shared__ cplx tw[N];
   __shared__ cplx buf[LP * LS];
   const int y0 = blockIdx.x * TY, x = blockIdx.y;
-  load_tw<N>(tw, twg);
+  load_tw<N, TA>(tw, twg);
   const long e0 = ((long)x * N + y0) * N;  // the tile is P consecutive voxels
+  auto inb = [&](int p) { return P % TA == 0 || p < P; };
   double f[HASK ? PT : 1][9];
   if constexpr (HASK) {
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
-      const int p = threadIdx.x + u * kT;
-      if (P % kT == 0 || p < P) {
+      const int p = threadIdx.x + u * TA;
+      if (inb(p)) {
 #pragma unroll
         for (int d = 0; d < 9; ++d) f[u][d] = F[d * V + e0 + p];
       }
     }
   }
+  // K4 rows 9c..9c+8 (or component c of F) at this thread's points
+  auto load_c = [&](int c, double (&k)[PT][NK]) {
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int p = threadIdx.x + u * TA;
+      if (inb(p)) {
+#pragma unroll
+        for (int d = 0; d < NK; ++d)
+          k[u][d] = HASK ? K4[(9 * c + d) * V + e0 + p] : F[c * V + e0 + p];
+      }
+    }
+  };
+  double kc[PT][NK], kn[PT][NK];
+  load_c(0, kc);
   double* bd = reinterpret_cast<double*>(buf);
+#pragma unroll
   for (int c = 0; c < 9; ++c) {
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
-      const int p = threadIdx.x + u * kT;
-      if (P % kT == 0 || p < P) {
+      const int p = threadIdx.x + u * TA;
+      if (inb(p)) {
         double v;
         if constexpr (HASK) {
           double t[9];
 #pragma unroll
-          for (int d = 0; d < 9; ++d) t[d] = K4[(9 * c + d) * V + e0 + p] * f[u][d];
+          for (int d = 0; d < 9; ++d) t[d] = kc[u][d] * f[u][d];
           v = t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
         } else {
-          v = F[c * V + e0 + p];
+          v = kc[u][0];
         }
         const int yy = p / N, z = p - yy * N;
         bd[2 * ((yy >> 1) * LS + pidx(z)) + (yy & 1)] = v;
       }
     }
+    if (c < 8) load_c(c + 1, kn);
     __syncthreads();
-    fft<N, LP, true>(buf, tw);
+    fft<N, LP, true, TA>(buf, tw);
     // Z = FFT(a + i b): A_k = (Z_k + conj Z_{N-k})/2, B_k = -i (Z_k - conj Z_{N-k})/2
-    for (int q = threadIdx.x; q < LP * NZ; q += kT) {
+    for (int q = threadIdx.x; q < LP * NZ; q += TA) {
       const int k = q / LP, m = q - k * LP;
       const cplx* line = buf + m * LS;
       const cplx zk = line[pidx(k)], zn = line[pidx(k == 0 ? 0 : N - k)];
@@ -254,6 +281,12 @@ __global__ __launch_bounds__(kT) void kA_zfwd(const double* __restrict__ F,
       o[1] = cplx{0.5 * (zk.im + zn.im), -0.5 * (zk.re - zn.re)};
     }
     __syncthreads();
+    if (c < 8) {
+#pragma unroll
+      for (int u = 0; u < PT; ++u)
+#pragma unroll
+        for (int d = 0; d < NK; ++d) kc[u][d] = kn[u][d];
+    }
   }
 }
 
@@ -298,35 +331,42 @@ __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* _
 }
 
 // ---- pass C: FFT along x, Green projection, inverse FFT along x ------------
-// grid (N, NZ): blockIdx.x = y, blockIdx.y = kz
+// YB consecutive y lines of all nine components per block.
+// grid (N/YB, NZ): blockIdx.x = y / YB, blockIdx.y = kz
+constexpr int ylines_x(int) { return 1; }
+
 template <int N>
 __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __restrict__ twg,
                                            int gamma_mode, const int* __restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int LS = lstride(N), NZ = N / 2 + 1;
+  constexpr int LS = lstride(N), NZ = N / 2 + 1, YB = ylines_x(N);
   __shared__ cplx tw[N];
-  __shared__ cplx buf[9 * LS];
-  const int y = blockIdx.x, kz = blockIdx.y;
+  __shared__ cplx buf[9 * YB * LS];
+  const int y0 = blockIdx.x * YB, kz = blockIdx.y;
   load_tw<N>(tw, twg);
-  for (int i = threadIdx.x; i < 9 * N; i += kT) {
-    const int c = i / N, x = i - c * N;
-    buf[c * LS + pidx(x)] = T2[(((long)c * NZ + kz) * N + y) * N + x];
+  // line (c*YB + t) holds component c of y line y0 + t
+  for (int i = threadIdx.x; i < 9 * YB * N; i += kT) {
+    const int l = i / N, x = i - l * N;
+    const int c = l / YB, t = l - c * YB;
+    buf[l * LS + pidx(x)] = T2[(((long)c * NZ + kz) * N + y0 + t) * N + x];
   }
   __syncthreads();
-  fft<N, 9, true>(buf, tw);
-  for (int x = threadIdx.x; x < N; x += kT) {
+  fft<N, 9 * YB, true>(buf, tw);
+  for (int i = threadIdx.x; i < YB * N; i += kT) {
+    const int t = i / N, x = i - t * N;
     cplx A[9], out[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) A[c] = buf[c * LS + pidx(x)];
-    project9(N, gamma_mode, x, y, kz, A, out);
+    for (int c = 0; c < 9; ++c) A[c] = buf[(c * YB + t) * LS + pidx(x)];
+    project9(N, gamma_mode, x, y0 + t, kz, A, out);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) buf[c * LS + pidx(x)] = out[c];
+    for (int c = 0; c < 9; ++c) buf[(c * YB + t) * LS + pidx(x)] = out[c];
   }
   __syncthreads();
-  fft<N, 9, false>(buf, tw);
-  for (int i = threadIdx.x; i < 9 * N; i += kT) {
-    const int c = i / N, x = i - c * N;
-    T2[(((long)c * NZ + kz) * N + y) * N + x] = buf[c * LS + pidx(x)];
+  fft<N, 9 * YB, false>(buf, tw);
+  for (int i = threadIdx.x; i < 9 * YB * N; i += kT) {
+    const int l = i / N, x = i - l * N;
+    const int c = l / YB, t = l - c * YB;
+    T2[(((long)c * NZ + kz) * N + y0 + t) * N + x] = buf[l * LS + pidx(x)];
   }
 }
 
@@ -370,11 +410,11 @@ void launch(const SpecGeom& g, const double* F, const double* K4, double* out, c
   static_assert(N % TY == 0 && TY % 2 == 0 && N % TX == 0, "tile shapes");
   const int* sk = g.skip;
   if (K4)
-    kA_zfwd<N, true><<<dim3(N / TY, N), kT, 0, s>>>(F, K4, T1, tw, sk);
+    kA_zfwd<N, true><<<dim3(N / TY, N), threads_z(N, true), 0, s>>>(F, K4, T1, tw, sk);
   else
-    kA_zfwd<N, false><<<dim3(N / TY, N), kT, 0, s>>>(F, nullptr, T1, tw, sk);
+    kA_zfwd<N, false><<<dim3(N / TY, N), threads_z(N, false), 0, s>>>(F, nullptr, T1, tw, sk);
   kB_y<N, true><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T1, T2, tw, sk);
-  kC_x<N><<<dim3(N, NZ), kT, 0, s>>>(T2, tw, g.gamma_mode, sk);
+  kC_x<N><<<dim3(N / ylines_x(N), NZ), kT, 0, s>>>(T2, tw, g.gamma_mode, sk);
   kB_y<N, false><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T2, T1, tw, sk);
   kD_zinv<N><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk);
 }
