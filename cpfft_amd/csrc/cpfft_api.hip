@@ -55,6 +55,10 @@ struct cpfft_ctx {
   double* hres = nullptr;  // pinned host mirror
   CGScalars* dcg = nullptr;  // device scalars of the running CG solve
   CGScalars* hcg = nullptr;  // pinned host mirror
+  // tangent_homo's nine simultaneous solves: x, r, p, q per right-hand side
+  double* th[4][9] = {};
+  CGScalars* dcg9 = nullptr;
+  CGScalars* hcg9 = nullptr;
   void* orients = nullptr;
   int32_t* oid = nullptr;
   long long* dcnt = nullptr;
@@ -92,6 +96,11 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
     if (b) hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
   if (c->hcg) (void)hipHostFree(c->hcg);
+  if (c->hcg9) (void)hipHostFree(c->hcg9);
+  for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 9; ++i)
+      if (c->th[a][i]) (void)hipFree(c->th[a][i]);
+  if (c->dcg9) (void)hipFree(c->dcg9);
   for (int f = 0; f < cpfft_ctx::kFam; ++f)
     for (int k = 0; k < 2; ++k)
       for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
@@ -190,6 +199,10 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcg, sizeof(CGScalars)));
   HIP_OK(hipHostMalloc(&c->hcg, sizeof(CGScalars), hipHostMallocDefault));
+  for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 9; ++i) HIP_OK(hipMalloc(&c->th[a][i], 9 * V * sizeof(double)));
+  HIP_OK(hipMalloc(&c->dcg9, 9 * sizeof(CGScalars)));
+  HIP_OK(hipHostMalloc(&c->hcg9, 9 * sizeof(CGScalars), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcnt, 8 * sizeof(long long)));
   HIP_OK(hipMemset(c->dcnt, 0, 8 * sizeof(long long)));
   // twiddles exp(-2 pi i k/N), computed in long double
@@ -464,24 +477,82 @@ extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
   return (int)(after[3] - before[3]);
 }
 
-// tangent_homo (tangent_homo.f:11-73)
+// tangent_homo (tangent_homo.f:11-73).  Column i of C_homo is
+// mean_e K4(e) : (E_i + dF_i(e)) with dF_i the CG solution of
+// G(K4 : dF_i) = -G(K4 : E_i).  The nine solves run together: one pass over
+// K4 forms K4 : p_i for every solve still running, then each solve applies
+// the projection and its own CG update.  Each solve's arithmetic is that of
+// fftPcg on its own, so iterates and iteration counts are unchanged.
 extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
-  const long V = c->V;
-  double* C = c->fields[CPFFT_WORK];
-  double* b = c->tmp9;
+  const long V = c->V, n = 9 * V;
+  const int maxIter = 1000;
+  const double eps = 2.220446049250313e-16;
+  if (tol <= eps || tol >= 1.0) return fail(-1, "cpfft_tangent_homo: improper tolerance");
+  double* const* X = c->th[0];
+  double* const* R = c->th[1];
+  double* const* Pd = c->th[2];
+  double* const* Q = c->th[3];
+  const double* K4 = c->fields[CPFFT_K4];
+  // right-hand sides b_i = -G(K4 : E_i) in R[i]
   for (int i = 0; i < 9; ++i) {
-    launch_gather_k4_col(V, i, c->fields[CPFFT_K4], C, c->stream);
+    launch_gather_k4_col(V, i, K4, Q[i], c->stream);
     tstart(c, 3);
-    launch_gkdf(c->geom, C, nullptr, b, c->spec, c->tw, c->stream);
+    launch_gkdf(c->geom, Q[i], nullptr, R[i], c->spec, c->tw, c->stream);
     tstop(c, 3);
     c->gkdf_calls++;
-    launch_scal(9 * V, -1.0, b, c->stream);
-    int it;
-    double rel;
-    int rc = cpfft_pcg(c, b, C, tol, &it, &rel);
-    if (rc < 0) return rc;
-    launch_add_identity_comp(V, i, C, c->stream);
-    launch_k4_dot9(V, c->fields[CPFFT_K4], C, c->partials, c->dres, c->stream);
+    launch_scal(n, -1.0, R[i], c->stream);
+    launch_dot(R[i], R[i], n, c->partials, c->dres + i, c->stream);
+  }
+  int rc = read_res(c, 9);
+  if (rc) return rc;
+  double n2b[9];
+  for (int i = 0; i < 9; ++i) {
+    const double bb = c->hres[i];
+    n2b[i] = sqrt(bb);
+    CGScalars h{};
+    h.rr = bb;
+    h.tolb = tol * n2b[i];
+    h.tol = tol;
+    h.resnorm = n2b[i];
+    h.maxit = maxIter;
+    h.done = (n2b[i] <= eps) ? 1 : 0;  // zero right-hand side: dF_i = 0
+    c->hcg9[i] = h;
+    launch_scal(n, 0.0, X[i], c->stream);
+    launch_copy(n, R[i], Pd[i], c->stream);
+  }
+  HIP_OK(hipMemcpyAsync(c->dcg9, c->hcg9, 9 * sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
+  Ptr9 pp;
+  for (int i = 0; i < 9; ++i) {
+    pp.p[i] = Pd[i];
+    pp.q[i] = Q[i];
+  }
+  const int chunk = 8;
+  for (;;) {
+    for (int j = 0; j < chunk; ++j) {
+      launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream);
+      for (int i = 0; i < 9; ++i) {
+        SpecGeom g = c->geom;
+        g.skip = &c->dcg9[i].done;
+        launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);  // in place: pass A reads all of Q[i] first
+        launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
+        launch_cg_dir_dev(n, c->dcg9 + i, R[i], Pd[i], c->stream);
+      }
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(c->hcg9, c->dcg9, 9 * sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    bool all = true;
+    for (int i = 0; i < 9; ++i) all &= c->hcg9[i].done != 0;
+    if (all) break;
+  }
+  for (int i = 0; i < 9; ++i) {
+    c->cg_iters += c->hcg9[i].k;
+    c->gkdf_calls += c->hcg9[i].k;
+    if (c->hcg9[i].done == 2) return fail(-2, "cpfft_tangent_homo: CG did not converge within 1000 iterations");
+  }
+  for (int i = 0; i < 9; ++i) {
+    launch_add_identity_comp(V, i, X[i], c->stream);
+    launch_k4_dot9(V, K4, X[i], c->partials, c->dres, c->stream);
     if ((rc = read_res(c, 9))) return rc;
     for (int j = 0; j < 9; ++j) C_homo[9 * j + i] = c->hres[j];
   }

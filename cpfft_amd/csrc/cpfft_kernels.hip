@@ -143,11 +143,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_fwd(SpecGeom g, const double* 
       // ddot42n: C_r = t1 + (((t2+t6)+(t4+t8)) + ((t3+t7)+(t5+t9)))
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
-        double t[9];
+        double k[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) t[c] = K4[(9 * r + c) * V + e] * f[c];
-        double v = t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
-        buf[r * N + z] = cplx{v, 0.0};
+        for (int c = 0; c < 9; ++c) k[c] = K4[(9 * r + c) * V + e];
+        buf[r * N + z] = cplx{ddot42n_row(k, f), 0.0};
       }
     } else {
 #pragma unroll
@@ -540,6 +539,35 @@ void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, dou
 }
 void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s) {
   k_cg_dir_s<<<grid_for(n), kBlock, 0, s>>>(n, cs, r, p);
+}
+
+// ------------------------------------------ multi-RHS K4 contraction
+// q_i = K4 : p_i for the right-hand sides i < nrhs whose solve is still
+// running (tangent_homo's nine solves share K4, which is read once).
+__global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const double* __restrict__ K4,
+                                                              Ptr9 pp, const CGScalars* __restrict__ cs,
+                                                              int nrhs) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= V) return;
+  bool any = false;
+  for (int i = 0; i < nrhs; ++i) any |= cs[i].done == 0;
+  if (!any) return;
+  double k[81];
+#pragma unroll
+  for (int j = 0; j < 81; ++j) k[j] = K4[j * V + e];
+  for (int i = 0; i < nrhs; ++i) {
+    if (cs[i].done) continue;
+    double f[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) f[c] = pp.p[i][c * V + e];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) pp.q[i][r * V + e] = ddot42n_row(k + 9 * r, f);
+  }
+}
+
+void launch_k4_contract_multi(long V, const double* K4, const Ptr9& pp, const CGScalars* cs,
+                              int nrhs, hipStream_t s) {
+  k_k4_contract_multi<<<(unsigned)((V + kBlock - 1) / kBlock), kBlock, 0, s>>>(V, K4, pp, cs, nrhs);
 }
 
 // ------------------------------------------------------- stress update

@@ -257,10 +257,7 @@ __global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* __re
       if (inb(p)) {
         double v;
         if constexpr (HASK) {
-          double t[9];
-#pragma unroll
-          for (int d = 0; d < 9; ++d) t[d] = kc[u][d] * f[u][d];
-          v = t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
+          v = ddot42n_row(kc[u], f[u]);
         } else {
           v = kc[u][0];
         }

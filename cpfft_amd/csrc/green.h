@@ -14,6 +14,18 @@
 
 namespace cpfft {
 
+// One row of ddot42n (G_K_dF.f:241-268): sum_c K[c] F[c] with the
+// reference's pairwise order t1 + (((t2+t6)+(t4+t8)) + ((t3+t7)+(t5+t9)));
+// products are rounded before the sum (no contraction) wherever it is used,
+// so every kernel that forms K4:F gets the same bits.
+__device__ inline double ddot42n_row(const double* K, const double* F) {
+#pragma clang fp contract(off)
+  double t[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) t[d] = K[d] * F[d];
+  return t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
+}
+
 // out += w * q_j (A q)_i / |q|^2
 __device__ inline void gamma_acc(const double q[3], const cplx A[9], cplx out[9], double w) {
   double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];

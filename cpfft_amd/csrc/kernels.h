@@ -52,6 +52,15 @@ void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, dou
 // p = r + beta p with beta from *cs (skipped once cs->done is set)
 void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s);
 
+// up to nine field pointers (multi-RHS solves)
+struct Ptr9 {
+  const double* p[9];
+  double* q[9];
+};
+// q_i = K4 : p_i (ddot42n) for every i < nrhs with cs[i].done == 0
+void launch_k4_contract_multi(long V, const double* K4, const Ptr9& pp, const CGScalars* cs,
+                              int nrhs, hipStream_t s);
+
 // reductions: result -> d_out[0..nres) on device (two-stage, fixed order)
 void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,
                 hipStream_t s);
