@@ -5,6 +5,11 @@ A step is one converged load step of the FFT_nr3 loop (mixed-BC uniaxial
 tension of the 128^3, 256-grain FCC polycrystal), driven by the Fortran
 driver over the HIP operators.  W warm-up steps run first (they include the
 initial elastic call and homogenised tangent), then K steps are timed.
+The headline value is the plain method.  A second solver repeats the run
+with the learned warm start: it logs training pairs during the W warm-up
+steps, trains the network on the GPU, and times the same K steps
+warm-started; its rate and its averaged-stress difference from the plain run
+are reported beside the headline.
 
 Prints one JSON line.  The roofline object describes G_K_dF with K4 (the
 CG matrix-vector product, five kernels launched as one group), whose device
@@ -97,13 +102,70 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
     }
 
 
+def timed_steps(s, wl, BC_all, isNBC, W, K):
+    """Time load steps W+1..W+K on a solver that has run steps 1..W;
+    returns (wall, rc, Pbar, cnt, times, counter deltas, kernel avg ms, counts)."""
+    c0 = s.counters()
+    s.set_timing(True)
+    device_sync()
+    s.sync()
+    t0 = time.perf_counter()
+    rc, Pbar, cnt, times = s.compute(BC_all[W:W + K], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter,
+                                     first_step=W + 1)
+    s.sync()
+    device_sync()
+    t1 = time.perf_counter()
+    avg, nlaunch = s.kernel_times()
+    c1 = s.counters()
+    return t1 - t0, rc, Pbar, cnt, times, [b - a for a, b in zip(c0, c1)], avg, nlaunch
+
+
+def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
+    """The same W + K steps with the learned warm start (trained on pairs
+    logged during the W warm-up steps)."""
+    V = wl.N ** 3
+    s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    stride = max(1, (W * V) // (1 << 21))  # at most ~2M logged pairs
+    s.mlp_init(seed=1, capacity=W * ((V + stride - 1) // stride), batch=4096)
+    rc, _, _, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1,
+                            mlp_log_last=W, mlp_log_stride=stride)
+    if rc != 0:
+        raise SystemExit(f"warm-start warm-up failed with code {rc}")
+    t = time.perf_counter()
+    iters = 3000
+    loss = s.mlp_train(iters, lr=2e-3)
+    t_train = time.perf_counter() - t
+    s.mlp_warm_start(True)
+    wall, rc, Pbar, cnt, times, d, avg, _ = timed_steps(s, wl, BC_all, isNBC, W, K)
+    if rc != 0:
+        raise SystemExit(f"warm-started steps failed with code {rc}")
+    st = s.mlp_stats()
+    s.close()
+    n_des = d[2]
+    lw = (d[3] + d[4]) / max(1, n_des * V)
+    lp = (d_plain[3] + d_plain[4]) / max(1, d_plain[2] * V)
+    return {
+        "value": K / wall, "unit": "load steps/s", "ms_per_step": 1e3 * wall / K,
+        "voxel_updates_per_s": V * n_des / wall,
+        "drive_eps_sig_ms": avg[1],
+        "local_iterations_per_voxel_update": lw,
+        "plain_local_iterations_per_voxel_update": lp,
+        "Pbar_max_rel_diff_vs_plain": float(abs(Pbar - Pbar_plain).max() / abs(Pbar_plain).max()),
+        "newton_per_step": float(cnt[:, 0].mean()),
+        "training": {"pairs": int(st[0]), "log_stride": stride, "adam_steps": iters,
+                     "seconds": t_train, "final_mse_normalised": loss},
+        "fallbacks_to_plain_start": int(st[3]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3_128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -124,30 +186,17 @@ def main():
     if rc != 0:
         raise SystemExit(f"warm-up failed with code {rc}")
     print(f"[bench] warm-up done: counters {cw.tolist()}", flush=True)
-    c0 = s.counters()
-    s.set_timing(True)
-    device_sync()
-    s.sync()
-    t0 = time.perf_counter()
-    rc, Pbar, cnt, times = s.compute(BC_all[W:nstep], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter,
-                                     first_step=W + 1)
-    s.sync()
-    device_sync()
-    t1 = time.perf_counter()
+    wall, rc, Pbar, cnt, times, d, avg, nlaunch = timed_steps(s, wl, BC_all, isNBC, W, K)
     if rc != 0:
         raise SystemExit(f"timed steps failed with code {rc}")
-    avg, nlaunch = s.kernel_times()
-    c1 = s.counters()
-    wall = t1 - t0
     V = wl.N ** 3
-    d = [b - a for a, b in zip(c0, c1)]
     n_des = d[2]
     n_gkdf = d[0]
     steps_per_s = K / wall
     gk_ms = avg[0]
     achieved = GKDF_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
     out = {
-        "metric": "converged load steps/s at 128^3 (plain method)",
+        "metric": "converged load steps/s at 128^3 (plain method; warm start in warm_start)",
         "value": steps_per_s,
         "unit": "load steps/s",
         "n_gpus": 1,
@@ -192,6 +241,9 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, s, (n_gkdf / K, n_des / K))
     s.close()
+    if not args.no_warm_start and wl.material.slip_set == 1:
+        print("[bench] warm-start leg", flush=True)
+        out["warm_start"] = warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar, d)
     print(json.dumps(out), flush=True)
 
 

@@ -43,7 +43,12 @@ EXPORTS = [
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
     "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_counters",
     "cpfft_kernel_times", "cpfft_set_timing",
+    "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
+    "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_stats",
 ]
+
+MLP_IN, MLP_HID, MLP_OUT = 42, 128, 12
+MLP_NPARAM, MLP_NNORM = 23564, 108
 
 _lib = None
 _drv = None
@@ -82,6 +87,15 @@ def lib():
         L.cpfft_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong)]
         L.cpfft_kernel_times.argtypes = [_vp, _dp]
         L.cpfft_set_timing.argtypes = [_vp, ctypes.c_int]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.cpfft_mlp_init.argtypes = [_vp, ctypes.c_uint64, ctypes.c_long, ctypes.c_int]
+        L.cpfft_mlp_log.argtypes = [_vp, ctypes.c_int, ctypes.c_long]
+        L.cpfft_mlp_train.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
+        L.cpfft_mlp_warm_start.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_mlp_params.argtypes = [_vp, _fp, ctypes.c_int]
+        L.cpfft_mlp_forward.argtypes = [_vp, _fp, ctypes.c_int, _fp]
+        L.cpfft_mlp_grad.argtypes = [_vp, _fp, _fp, ctypes.c_int, _fp, _dp]
+        L.cpfft_mlp_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong)]
         _lib = L
     return _lib
 
@@ -95,7 +109,8 @@ def driver():
             raise RuntimeError(f"{DRIVER} is missing: build it first (python -m cpfft_amd.build)")
         D = ctypes.CDLL(DRIVER)
         D.cpfft_compute.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double,
-                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _ip, _dp]
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _ip, _dp,
+                                    ctypes.c_int, ctypes.c_int]
         _drv = D
     return _drv
 
@@ -103,6 +118,11 @@ def driver():
 def _p(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)
+
+
+def _pf(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 class CpfftError(RuntimeError):
@@ -227,13 +247,56 @@ class Solver:
         lib().cpfft_kernel_times(self.h, _p(out))
         return out[:4], out[4:].astype(int)
 
+    # ---- learned warm start (no counterpart in the reference) ----------
+    def mlp_init(self, seed=1, capacity=1 << 20, batch=4096):
+        _check(lib().cpfft_mlp_init(self.h, seed, capacity, batch), "cpfft_mlp_init")
+
+    def mlp_train(self, iters, lr=1e-3, seed=7):
+        loss = ctypes.c_double(0.0)
+        _check(lib().cpfft_mlp_train(self.h, iters, lr, seed, ctypes.byref(loss)), "cpfft_mlp_train")
+        return loss.value
+
+    def mlp_warm_start(self, on):
+        _check(lib().cpfft_mlp_warm_start(self.h, int(on)), "cpfft_mlp_warm_start")
+
+    def mlp_params(self, new=None):
+        """Get (new=None) or set the flat parameters + normalisation vector."""
+        if new is None:
+            out = np.zeros(MLP_NPARAM + MLP_NNORM, dtype=np.float32)
+            _check(lib().cpfft_mlp_params(self.h, _pf(out), 1), "cpfft_mlp_params")
+            return out
+        buf = np.ascontiguousarray(new, dtype=np.float32)
+        assert buf.size == MLP_NPARAM + MLP_NNORM
+        _check(lib().cpfft_mlp_params(self.h, _pf(buf), 0), "cpfft_mlp_params")
+
+    def mlp_forward(self, Xn):
+        X = np.ascontiguousarray(Xn, dtype=np.float32)
+        Y = np.zeros((X.shape[0], MLP_OUT), dtype=np.float32)
+        _check(lib().cpfft_mlp_forward(self.h, _pf(X), X.shape[0], _pf(Y)), "cpfft_mlp_forward")
+        return Y
+
+    def mlp_grad(self, Xn, Tn):
+        X = np.ascontiguousarray(Xn, dtype=np.float32)
+        T = np.ascontiguousarray(Tn, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(lib().cpfft_mlp_grad(self.h, _pf(X), _pf(T), X.shape[0], _pf(g), ctypes.byref(loss)),
+               "cpfft_mlp_grad")
+        return g, loss.value
+
+    def mlp_stats(self):
+        out = (ctypes.c_longlong * 4)()
+        _check(lib().cpfft_mlp_stats(self.h, out), "cpfft_mlp_stats")
+        return list(out)
+
     def compute(self, BC_all, isNBC, tolPCG=1e-8, tolNR=1e-6, maxIter=25, verbose=False,
-                first_step=1):
+                first_step=1, mlp_log_last=0, mlp_log_stride=1):
         """The `compute` command: drive_eps_sig(1,0) then the FFT_nr3 load-step
         loop, run by the Fortran driver, for the steps whose boundary values
         BC_all holds, numbered from first_step (a first_step above 1
-        continues the previous call).  Returns (rc, Pbar[nstep,9],
-        counters[nstep,4], times[3])."""
+        continues the previous call).  Steps up to mlp_log_last log
+        warm-start training pairs (every mlp_log_stride-th voxel).  Returns
+        (rc, Pbar[nstep,9], counters[nstep,4], times[3])."""
         BC = np.ascontiguousarray(BC_all, dtype=np.float64)
         nstep = BC.shape[0]
         nbc = np.ascontiguousarray(isNBC, dtype=np.int32)
@@ -243,5 +306,5 @@ class Solver:
         rc = driver().cpfft_compute(self.h, first_step, nstep, _p(BC), nbc.ctypes.data_as(_ip),
                                     tolPCG, tolNR,
                                     maxIter, int(verbose), _p(Pbar), cnt.ctypes.data_as(_ip),
-                                    _p(times))
+                                    _p(times), int(mlp_log_last), int(mlp_log_stride))
         return rc, Pbar, cnt, times

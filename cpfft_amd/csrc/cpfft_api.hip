@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <random>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -59,6 +60,13 @@ struct cpfft_ctx {
   double* th[4][9] = {};
   CGScalars* dcg9 = nullptr;
   CGScalars* hcg9 = nullptr;
+  // learned warm start
+  MlpBuffers mlp;
+  bool mlp_ready = false;  // buffers allocated
+  int mlp_on = 0;
+  long mlp_count = 0;      // logged pairs
+  long mlp_steps = 0;      // Adam steps taken
+  long long ws_updates = 0;
   void* orients = nullptr;
   int32_t* oid = nullptr;
   long long* dcnt = nullptr;
@@ -101,6 +109,14 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
     for (int i = 0; i < 9; ++i)
       if (c->th[a][i]) (void)hipFree(c->th[a][i]);
   if (c->dcg9) (void)hipFree(c->dcg9);
+  {
+    MlpBuffers& b = c->mlp;
+    void* mb[] = {b.params, b.pbf, b.grads, b.m, b.v, b.norm, b.logX, b.logT, b.Xb, b.Xf, b.Tb,
+                  b.H1, b.H2, b.Y, b.dY, b.dZ1, b.dZ2, b.H1b, b.H2b, b.dYb, b.dZ1b, b.dZ2b,
+                  b.part, b.ws};
+    for (void* q : mb)
+      if (q) (void)hipFree(q);
+  }
   for (int f = 0; f < cpfft_ctx::kFam; ++f)
     for (int k = 0; k < 2; ++k)
       for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
@@ -454,20 +470,42 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   return 0;
 }
 
+// predicted slip increments of every voxel into c->mlp.ws (network inference
+// in chunks of the work-buffer size)
+static void mlp_predict(cpfft_ctx* c, int step) {
+  MlpBuffers& b = c->mlp;
+  mlp_weights_to_bf16(b, c->stream);
+  for (long e0 = 0; e0 < c->V; e0 += b.rows) {
+    const long nr = std::min((long)b.rows, c->V - e0);
+    launch_mlp_features(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
+                        c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                        nullptr, e0, 1, nr, b.Xf, nullptr, c->stream);
+    mlp_norm_inputs(b, nr, b.Xf, b.Xb, c->stream);
+    mlp_forward(b, (int)nr, b.Xb, b.Y, c->stream);
+    launch_mlp_denorm(c->V, e0, nr, b.Y, b.norm, b.ws, c->stream);
+  }
+}
+
 extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
   long long before[8];
   HIP_OK(hipMemcpyAsync(before, c->dcnt, sizeof before, hipMemcpyDeviceToHost, c->stream));
   tstart(c, 1);
+  const float* ws = nullptr;
+  if (c->mlp_on && iiter > 0 && c->nslip == 12) {
+    mlp_predict(c, step);
+    ws = c->mlp.ws;
+    c->ws_updates += c->V;
+  }
   if (c->nslip == 12)
     launch_drive_eps_sig<12>(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
-                             c->fields[CPFFT_K4], c->dcnt, c->stream);
+                             c->fields[CPFFT_K4], ws, c->dcnt, c->stream);
   else
     launch_drive_eps_sig<24>(c->V, c->mp, (const Orient<24>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
-                             c->fields[CPFFT_K4], c->dcnt, c->stream);
+                             c->fields[CPFFT_K4], nullptr, c->dcnt, c->stream);
   tstop(c, 1);
   HIP_OK(hipGetLastError());
   long long after[8];
@@ -578,5 +616,176 @@ extern "C" int cpfft_counters(cpfft_ctx* c, long long out[8]) {
   out[5] = d[2];
   out[6] = d[3];
   out[7] = d[4];
+  return 0;
+}
+
+// ------------------------------------------------------ learned warm start
+extern "C" int cpfft_mlp_init(cpfft_ctx* c, uint64_t seed, long capacity, int batch) {
+  if (!c) return fail(-1, "cpfft_mlp_init: null context");
+  if (c->nslip != 12) return fail(-1, "cpfft_mlp_init: the warm start is defined for fcc (12 slip systems)");
+  if (capacity < 1 || batch < 16) return fail(-1, "cpfft_mlp_init: capacity >= 1 and batch >= 16 required");
+  MlpBuffers& b = c->mlp;
+  if (!c->mlp_ready) {
+    const int rows = std::max(batch, 65536);
+    b.rows = rows;
+    b.cap = capacity;
+    HIP_OK(hipMalloc(&b.params, kMlpParams * sizeof(float)));
+    HIP_OK(hipMalloc(&b.pbf, kMlpParams * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.grads, kMlpParams * sizeof(float)));
+    HIP_OK(hipMalloc(&b.m, kMlpParams * sizeof(float)));
+    HIP_OK(hipMalloc(&b.v, kMlpParams * sizeof(float)));
+    HIP_OK(hipMalloc(&b.norm, kMlpNorm * sizeof(float)));
+    HIP_OK(hipMalloc(&b.logX, capacity * kMlpIn * sizeof(float)));
+    HIP_OK(hipMalloc(&b.logT, capacity * kMlpOut * sizeof(float)));
+    HIP_OK(hipMalloc(&b.Xb, (long)rows * kMlpIn * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.Xf, (long)rows * kMlpIn * sizeof(float)));
+    HIP_OK(hipMalloc(&b.Tb, (long)rows * kMlpOut * sizeof(float)));
+    float** f32[] = {&b.H1, &b.H2, &b.dZ1, &b.dZ2};
+    for (float** q : f32) HIP_OK(hipMalloc(q, (long)rows * kMlpHid * sizeof(float)));
+    HIP_OK(hipMalloc(&b.Y, (long)rows * kMlpOut * sizeof(float)));
+    HIP_OK(hipMalloc(&b.dY, (long)rows * kMlpOut * sizeof(float)));
+    uint16_t** b16[] = {&b.H1b, &b.H2b, &b.dZ1b, &b.dZ2b};
+    for (uint16_t** q : b16) HIP_OK(hipMalloc(q, (long)rows * kMlpHid * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.dYb, (long)rows * kMlpOut * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.part, (long)mlp_loss_parts(rows) * sizeof(double)));
+    HIP_OK(hipMalloc(&b.ws, (long)kMlpOut * c->V * sizeof(float)));
+    c->mlp_ready = true;
+  } else if (batch > b.rows || capacity > b.cap) {
+    return fail(-1, "cpfft_mlp_init: a larger batch or log than the first call allocated");
+  }
+  // Glorot-uniform weights, zero biases, identity normalisation
+  std::vector<float> h(kMlpParams, 0.0f);
+  std::mt19937_64 rng(seed);
+  auto fill = [&](long off, int fan_out, int fan_in) {
+    const double a = sqrt(6.0 / (double)(fan_in + fan_out));
+    std::uniform_real_distribution<double> U(-a, a);
+    for (long i = 0; i < (long)fan_out * fan_in; ++i) h[off + i] = (float)U(rng);
+  };
+  fill(kOffW1, kMlpHid, kMlpIn);
+  fill(kOffW2, kMlpHid, kMlpHid);
+  fill(kOffW3, kMlpOut, kMlpHid);
+  HIP_OK(hipMemcpy(b.params, h.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(b.m, 0, kMlpParams * sizeof(float)));
+  HIP_OK(hipMemset(b.v, 0, kMlpParams * sizeof(float)));
+  std::vector<float> nrm(kMlpNorm, 0.0f);
+  for (int i = 0; i < kMlpIn; ++i) nrm[kMlpIn + i] = 1.0f;
+  for (int i = 0; i < kMlpOut; ++i) nrm[2 * kMlpIn + kMlpOut + i] = 1.0f;
+  HIP_OK(hipMemcpy(b.norm, nrm.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
+  c->mlp_count = 0;
+  c->mlp_steps = 0;
+  c->mlp_on = 0;
+  return 0;
+}
+
+extern "C" int cpfft_mlp_log(cpfft_ctx* c, int step, long stride) {
+  if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_log: cpfft_mlp_init first");
+  if (stride < 1) return fail(-1, "cpfft_mlp_log: stride must be >= 1");
+  MlpBuffers& b = c->mlp;
+  long rows = (c->V + stride - 1) / stride;
+  rows = std::min(rows, b.cap - c->mlp_count);
+  if (rows <= 0) return 0;  // log full
+  launch_mlp_features(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
+                      c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                      c->fields[CPFFT_STATE_NP1], 0, stride, rows, b.logX + c->mlp_count * kMlpIn,
+                      b.logT + c->mlp_count * kMlpOut, c->stream);
+  HIP_OK(hipGetLastError());
+  c->mlp_count += rows;
+  return 0;
+}
+
+extern "C" int cpfft_mlp_train(cpfft_ctx* c, int iters, double lr, uint64_t seed, double* loss) {
+  if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_train: cpfft_mlp_init first");
+  if (c->mlp_count < 1) return fail(-1, "cpfft_mlp_train: no logged pairs");
+  MlpBuffers& b = c->mlp;
+  const int rows = std::min<long>(4096, b.rows);
+  mlp_stats(b, c->mlp_count, c->stream);
+  for (int it = 0; it < iters; ++it) {
+    mlp_weights_to_bf16(b, c->stream);
+    mlp_gather_batch(b, rows, c->mlp_count, (uint32_t)seed, (uint32_t)(c->mlp_steps + it), c->stream);
+    mlp_forward(b, rows, b.Xb, b.Y, c->stream);
+    mlp_backward(b, rows, b.Xb, b.Tb, b.grads, b.part, c->stream);
+    mlp_adam(b, (float)lr, c->mlp_steps + it + 1, c->stream);
+  }
+  HIP_OK(hipGetLastError());
+  c->mlp_steps += iters;
+  const int np = mlp_loss_parts(rows);
+  std::vector<double> hp(np);
+  HIP_OK(hipMemcpyAsync(hp.data(), b.part, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  double acc = 0.0;
+  for (double v : hp) acc += v;
+  if (loss) *loss = acc / ((double)rows * kMlpOut);
+  return 0;
+}
+
+extern "C" int cpfft_mlp_warm_start(cpfft_ctx* c, int on) {
+  if (!c) return fail(-1, "cpfft_mlp_warm_start: null context");
+  if (on && !c->mlp_ready) return fail(-1, "cpfft_mlp_warm_start: cpfft_mlp_init first");
+  c->mlp_on = on ? 1 : 0;
+  return 0;
+}
+
+extern "C" int cpfft_mlp_params(cpfft_ctx* c, float* buf, int get) {
+  if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_params: cpfft_mlp_init first");
+  MlpBuffers& b = c->mlp;
+  if (get) {
+    HIP_OK(hipMemcpy(buf, b.params, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(buf + kMlpParams, b.norm, kMlpNorm * sizeof(float), hipMemcpyDeviceToHost));
+  } else {
+    HIP_OK(hipMemcpy(b.params, buf, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.norm, buf + kMlpParams, kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+static int mlp_upload_batch(cpfft_ctx* c, const float* Xn, const float* Tn, int n) {
+  MlpBuffers& b = c->mlp;
+  if (n < 1 || n > b.rows) return fail(-1, "cpfft_mlp: batch size out of range");
+  HIP_OK(hipMemcpy(b.Xf, Xn, (long)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+  mlp_to_bf16((long)n * kMlpIn, b.Xf, b.Xb, c->stream);
+  if (Tn) HIP_OK(hipMemcpy(b.Tb, Tn, (long)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice));
+  mlp_weights_to_bf16(b, c->stream);
+  return 0;
+}
+
+extern "C" int cpfft_mlp_forward(cpfft_ctx* c, const float* Xn, int n, float* Y) {
+  if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_forward: cpfft_mlp_init first");
+  int rc = mlp_upload_batch(c, Xn, nullptr, n);
+  if (rc) return rc;
+  mlp_forward(c->mlp, n, c->mlp.Xb, c->mlp.Y, c->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  HIP_OK(hipMemcpy(Y, c->mlp.Y, (long)n * kMlpOut * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cpfft_mlp_grad(cpfft_ctx* c, const float* Xn, const float* Tn, int n, float* grad,
+                              double* loss) {
+  if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_grad: cpfft_mlp_init first");
+  int rc = mlp_upload_batch(c, Xn, Tn, n);
+  if (rc) return rc;
+  MlpBuffers& b = c->mlp;
+  mlp_forward(b, n, b.Xb, b.Y, c->stream);
+  mlp_backward(b, n, b.Xb, b.Tb, b.grads, b.part, c->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
+  HIP_OK(hipMemcpy(grad, b.grads, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  const int np = mlp_loss_parts(n);
+  std::vector<double> hp(np);
+  HIP_OK(hipMemcpy(hp.data(), b.part, np * sizeof(double), hipMemcpyDeviceToHost));
+  double acc = 0.0;
+  for (double v : hp) acc += v;
+  if (loss) *loss = acc / ((double)n * kMlpOut);
+  return 0;
+}
+
+extern "C" int cpfft_mlp_stats(cpfft_ctx* c, long long out[4]) {
+  if (!c) return fail(-1, "cpfft_mlp_stats: null context");
+  long long d[8];
+  HIP_OK(hipMemcpy(d, c->dcnt, sizeof d, hipMemcpyDeviceToHost));
+  out[0] = c->mlp_count;
+  out[1] = c->mlp_steps;
+  out[2] = c->ws_updates;
+  out[3] = d[5];
   return 0;
 }

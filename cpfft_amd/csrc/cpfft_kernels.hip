@@ -581,9 +581,10 @@ __global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
                                                       double* __restrict__ snp1,
                                                       double* __restrict__ Pn1,
                                                       double* __restrict__ tan6,
+                                                      const float* __restrict__ ws,
                                                       long long* __restrict__ counters) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  Counters cnt{0, 0, 0, 0, 0};
+  Counters cnt{0, 0, 0, 0, 0, 0};
   if (e < V) {
     const Orient<NS>& o = orients[oid[e]];
     voxel_update<NS>(
@@ -592,11 +593,11 @@ __global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
         [&](int w) { return sn[w * V + e]; },
         [&](int w, double v) { snp1[w * V + e] = v; },
         [&](int c, double v) { Pn1[c * V + e] = v; },
-        [&](int c, double v) { tan6[c * V + e] = v; }, cnt);
+        [&](int c, double v) { tan6[c * V + e] = v; }, cnt, ws ? ws + e : nullptr, V);
   }
   // wavefront-level sums of the counters, one integer atomic per wavefront
-  long long c[5] = {cnt.pred_it, cnt.upd_it, cnt.cuts, cnt.failed, cnt.ls_evals};
-  for (int k = 0; k < 5; ++k) {
+  long long c[6] = {cnt.pred_it, cnt.upd_it, cnt.cuts, cnt.failed, cnt.ls_evals, cnt.ws_fallback};
+  for (int k = 0; k < 6; ++k) {
     long long v = c[k];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)&counters[k], (unsigned long long)v);
@@ -623,19 +624,73 @@ template <int NS>
 void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
                           const int32_t* orient_id, int step, int iter, const double* Fn,
                           const double* Fn1, const double* sn, double* snp1, double* Pn1,
-                          double* tan6, double* K4, long long* counters, hipStream_t s) {
+                          double* tan6, double* K4, const float* ws, long long* counters,
+                          hipStream_t s) {
   const int bs = 64;
   unsigned g = (unsigned)((V + bs - 1) / bs);
   k_drive_eps_sig<NS><<<g, bs, 0, s>>>(V, P, orients, orient_id, step, iter, Fn, Fn1, sn, snp1,
-                                       Pn1, tan6, counters);
+                                       Pn1, tan6, ws, counters);
   k_cep2A<<<g, bs, 0, s>>>(V, Fn, Fn1, snp1, tan6, K4);
+}
+
+// ----------------------------------------------- warm-start features
+// rows r < nrows of voxels e = e0 + r * stride: raw network inputs into
+// X[r][42]; with T != nullptr also the converged slip increments of state
+// n+1, as u_s (slip_to_u), into T[r][12] (training pairs)
+__global__ __launch_bounds__(64) void k_mlp_features(long V, MatParams P,
+                                                     const Orient<12>* __restrict__ orients,
+                                                     const int32_t* __restrict__ oid, int step,
+                                                     const double* __restrict__ Fn,
+                                                     const double* __restrict__ Fn1,
+                                                     const double* __restrict__ sn,
+                                                     const double* __restrict__ snp1, long e0,
+                                                     long stride, long nrows,
+                                                     float* __restrict__ X, float* __restrict__ T) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const long e = e0 + r * stride;
+  if (e >= V) return;
+  float f[kMlpIn];
+  voxel_features(P, orients[oid[e]], step,
+                 [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
+                 [&](int w) { return sn[w * V + e]; }, f);
+  for (int k = 0; k < kMlpIn; ++k) X[r * kMlpIn + k] = f[k];
+  if (T) {
+    const double dg = P.alter_mode ? P.eps_dot_0_y * P.dt : 1.0;
+    for (int k = 0; k < kMlpOut; ++k) T[r * kMlpOut + k] = (float)slip_to_u(P, dg, snp1[(23 + k) * V + e]);
+  }
+}
+
+// predicted slip increments (as u_s) of rows r < nrows (voxels e0 + r)
+// from the normalised network output
+__global__ void k_mlp_denorm(long V, long e0, long nrows, const float* __restrict__ Y,
+                             const float* __restrict__ nrm, float* __restrict__ ws) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= nrows || e0 + r >= V) return;
+  const float* mt = nrm + 2 * kMlpIn;
+  const float* st = mt + kMlpOut;
+  for (int k = 0; k < kMlpOut; ++k) ws[k * V + e0 + r] = Y[r * kMlpOut + k] * st[k] + mt[k];
+}
+
+void launch_mlp_features(long V, const MatParams& P, const Orient<12>* orients, const int32_t* oid,
+                         int step, const double* Fn, const double* Fn1, const double* sn,
+                         const double* snp1, long e0, long stride, long nrows, float* X, float* T,
+                         hipStream_t s) {
+  k_mlp_features<<<(unsigned)((nrows + 63) / 64), 64, 0, s>>>(V, P, orients, oid, step, Fn, Fn1, sn,
+                                                              snp1, e0, stride, nrows, X, T);
+}
+void launch_mlp_denorm(long V, long e0, long nrows, const float* Y, const float* nrm, float* ws,
+                       hipStream_t s) {
+  k_mlp_denorm<<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(V, e0, nrows, Y, nrm, ws);
 }
 
 template void launch_drive_eps_sig<12>(long, const MatParams&, const Orient<12>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, double*, long long*, hipStream_t);
+                                       double*, double*, double*, double*, const float*,
+                                       long long*, hipStream_t);
 template void launch_drive_eps_sig<24>(long, const MatParams&, const Orient<24>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, double*, long long*, hipStream_t);
+                                       double*, double*, double*, double*, const float*,
+                                       long long*, hipStream_t);
 
 }  // namespace cpfft

@@ -7,6 +7,7 @@
 #include "cpfft_common.h"
 #include "green.h"
 #include "mm10_core.h"
+#include "mlp.h"
 
 namespace cpfft {
 
@@ -83,11 +84,23 @@ void launch_k4_dot9(long V, const double* K4, const double* C, double* partials,
 
 // per-voxel update: stress update (state n+1, P, material tangent into
 // tan6 [36][V]) followed by its pull-back to K4 = dP/dF
+// ws: predicted slip increments [12][V] for the warm start, or nullptr
 template <int NS>
 void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
                           const int32_t* orient_id, int step, int iter, const double* Fn,
                           const double* Fn1, const double* sn, double* snp1, double* Pn1,
-                          double* tan6, double* K4, long long* counters, hipStream_t s);
+                          double* tan6, double* K4, const float* ws, long long* counters,
+                          hipStream_t s);
+
+// warm-start network inputs of voxels e0 + r*stride (r < nrows) into
+// X[r][42]; with T also their converged slip increments (state n+1)
+void launch_mlp_features(long V, const MatParams& P, const Orient<12>* orients, const int32_t* oid,
+                         int step, const double* Fn, const double* Fn1, const double* sn,
+                         const double* snp1, long e0, long stride, long nrows, float* X, float* T,
+                         hipStream_t s);
+// ws[k][e0 + r] = Y[r][k] * sd_t[k] + mean_t[k]
+void launch_mlp_denorm(long V, long e0, long nrows, const float* Y, const float* nrm, float* ws,
+                       hipStream_t s);
 
 int reduce_partials_count(long n);
 

@@ -48,6 +48,7 @@ struct Orient {
 
 struct Counters {
   int pred_it, upd_it, cuts, failed, ls_evals;
+  int ws_fallback;  // warm-started solves that failed and were redone from the plain start
 };
 
 // ---------------------------------------------------------------- helpers
@@ -564,10 +565,12 @@ CPFFT_HD inline void setup_point(const MatParams& P, const double R[3][3], const
 
 // mm10_solve (mm10_a.f:2860-3295). x in/out; xprev receives the iterate at
 // which the last full Jacobian was formed. Returns fail.
+// x0w: optional starting point [stress, tt] (the warm start); without it
+// the predictor starts from x with tt extrapolated (mm10_a.f:2993-3023).
 template <int NS>
 CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const Point& pt,
                                 const StateN& n, double dtinc, double x[7], double xprev[7],
-                                double* tt_rate, Counters& cnt) {
+                                double* tt_rate, Counters& cnt, const double* x0w = nullptr) {
   const double cls = 1.0e-4, red = 0.5;
   const int mls = 10;
   int fail = 0;
@@ -595,7 +598,7 @@ CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const P
 #pragma unroll
     for (int k = 0; k < 6; ++k) cosang += d1[k] * d2[k];
     cosang = fmax(cosang, 0.0);
-    const double x2 = x[6] + cosang * n.tt_rate * dtinc;
+    double x2 = x[6] + cosang * n.tt_rate * dtinc;
     double x1[6], R1[6], sabs;
 #pragma unroll
     for (int k = 0; k < 6; ++k) x1[k] = x[k];
@@ -605,6 +608,26 @@ CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const P
     for (int k = 0; k < 6; ++k) dot += R1[k] * R1[k];
     double nR1 = sqrt(dot);
     inR1 = nR1;
+    if (x0w) {
+      // warm start: iterate from x0w when its residual is below the plain
+      // start's; the plain start's residual stays the reference of the
+      // relative stop test, so the solve stops at the accuracy the plain
+      // method asks for
+      double R1w[6], sabsw, dotw = 0.0;
+      form_R1<NS>(P, o, pt, n, x0w, x0w[6], R1w, &sabsw);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) dotw += R1w[k] * R1w[k];
+      if (dotw < dot) {
+        x2 = x0w[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          x1[k] = x0w[k];
+          R1[k] = R1w[k];
+        }
+        dot = dotw;
+        nR1 = sqrt(dot);
+      }
+    }
     int it = 0, pfail = 0;
     while (nR1 > P.atol1 && nR1 / inR1 > P.rtol1) {
       double A[6][6], dx[6][1], gJ[6];
@@ -941,14 +964,157 @@ CPFFT_HD inline void step_kinematics(const double Fn[3][3], const double Fn1[3][
   }
 }
 
+// ------------------------------------------------------- warm start
+// The network predicts each system's slip increment through its normalised
+// resolved shear u_s = rs_s / tt = sgn(g_s) (|g_s| / dg)^(1/n), a one-to-one
+// map of the slip increment g_s = dg |u_s|^n sgn(u_s) (Voce power law with
+// alter_mode) that stays of order one where g_s spans many decades.
+CPFFT_HD inline double slip_to_u(const MatParams& P, double dg, double g) {
+  return g == 0.0 ? 0.0 : sgn1(g) * pow(fabs(g) / dg, 1.0 / P.rate_n);
+}
+
+// Starting point of the local solve from predicted u_s:
+// tt0 from the discrete Voce law at S = sum_s dg |u_s|^n (closed form for
+// m = 1: tt = (tt_n + theta_0 S (1 + tau_y/tau_v)) / (1 + theta_0 S / tau_v));
+// the stress s0 minimises sum_s w_s (s . E m_s - tt0 u_s)^2 + lam |s - s_tr|^2
+// with weights w_s = |u_s|^n (the slipping systems), s_tr = s_n + C:D the
+// elastic trial stress and lam small, so that directions the slipping
+// systems do not constrain (and elastic voxels) take the trial stress.
+template <int NS>
+CPFFT_HD inline void warm_x0(const MatParams& P, const Orient<NS>& o, const Point& pt,
+                             const StateN& n, const float* ws, long wstride, double x0[7]) {
+  double S = 0.0;
+  for (int i = 0; i < NS; ++i) S += pt.dg * pw_nm2(P, fabs((double)ws[i * wstride])) *
+                                    ((double)ws[i * wstride] * (double)ws[i * wstride]);
+  double tt0;
+  if (P.m_is_one) {
+    tt0 = (n.tt + P.theta_0 * S * (1.0 + P.tau_y / P.tau_v)) / (1.0 + P.theta_0 * S / P.tau_v);
+  } else {
+    const double h = 1.0 - (n.tt - P.tau_y) / P.tau_v;
+    tt0 = n.tt + P.theta_0 * pow(fabs(h), P.voche_m) * sgn1(h) * S;
+  }
+  double A[6][6], b[6][1];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    b[a][0] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) A[a][c] = 0.0;
+  }
+  double wsum = 0.0;
+  for (int i = 0; i < NS; ++i) {
+    const double u = (double)ws[i * wstride];
+    const double w = pw_nm2(P, fabs(u)) * u * u;  // |u|^n
+    if (w == 0.0) continue;
+    double m[6], mc[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) m[k] = o.ms[i][k];
+    rotE(pt.tn, m, mc);
+    const double rs = tt0 * u;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      b[a][0] += w * rs * mc[a];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) A[a][c] += w * mc[a] * mc[c];
+      wsum += w * mc[a] * mc[a];
+    }
+  }
+  double c[6], st[6];
+  Cv6(o.C, pt.D, c);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) st[k] = n.stress[k] + c[k];
+  const double lam = 1.0e-6 * wsum + 1.0e-300;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    A[a][a] += lam;
+    b[a][0] += lam * st[a];
+  }
+  lu_solve<6, 1>(A, b);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) x0[k] = b[k][0];
+  x0[6] = tt0;
+}
+
+// Inputs of the warm-start network for one voxel (42 values, FCC): the
+// trial stress s_n + C:D rotated to the lattice frame (Rp_n^T S Rp_n as a
+// full 3x3, row-major), the slip resistance tt_n once per slip system (the
+// Voce law keeps a single one), the strain increment D in the lattice frame
+// (3x3 tensor, row-major), and the slip increments of the previous step
+// (as u_s, see slip_to_u).
+template <class LoadF, class LoadS>
+CPFFT_HD inline void voxel_features(const MatParams& P, const Orient<12>& o, int step, LoadF&& ldF,
+                                    LoadS&& ldn, float* feat) {
+  double uddt[6], rn1[3][3];
+  {
+    double Fn[3][3], Fn1[3][3], rnh[3][3], fnhinv[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        Fn[i][j] = ldF(0, 3 * i + j);
+        Fn1[i][j] = ldF(1, 3 * i + j);
+      }
+    step_kinematics(Fn, Fn1, rnh, rn1, fnhinv, uddt);
+  }
+  double sn[6], Rp[3][3], tt;
+  if (step == 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sn[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Rp[i][j] = (i == j) ? 1.0 : 0.0;
+    tt = P.tau_y + 1.0e-5;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sn[k] = ldn(k);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) Rp[i][j] = ldn(6 + 3 * j + i);
+    tt = ldn(15);
+  }
+  double c[6];
+  Cv6(o.C, uddt, c);
+  const double st[6] = {sn[0] + c[0], sn[1] + c[1], sn[2] + c[2],
+                        sn[3] + c[3], sn[4] + c[4], sn[5] + c[5]};
+  const double S[3][3] = {{st[0], st[3], st[5]}, {st[3], st[1], st[4]}, {st[5], st[4], st[2]}};
+  const double D[3][3] = {{uddt[0], 0.5 * uddt[3], 0.5 * uddt[5]},
+                          {0.5 * uddt[3], uddt[1], 0.5 * uddt[4]},
+                          {0.5 * uddt[5], 0.5 * uddt[4], uddt[2]}};
+  // lattice frame: Rp^T X Rp (tn = Rp^T maps lattice to unrotated)
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          a += Rp[k][i] * S[k][l] * Rp[l][j];
+          b += Rp[k][i] * D[k][l] * Rp[l][j];
+        }
+      feat[3 * i + j] = (float)a;
+      feat[21 + 3 * i + j] = (float)b;
+    }
+#pragma unroll
+  for (int s = 0; s < 12; ++s) feat[9 + s] = (float)tt;
+  const double dg = P.alter_mode ? P.eps_dot_0_y * P.dt : 1.0;
+  for (int s = 0; s < 12; ++s) feat[30 + s] = step == 1 ? 0.0f : (float)slip_to_u(P, dg, ldn(23 + s));
+}
+
 // Per-voxel stress update.  ldF(0|1, 3i+j) loads F_n | F_{n+1}; ldn(w)
 // loads state-n word w; stn1(w, v) stores state n+1; stP(3i+j, v) stores
 // P; stT(6a+b, v) stores the symmetrised material tangent (row-major 6x6)
 // that voxel_dPdF pulls back to dP/dF.
+// ws, if not null, holds predicted slip increments ws[s * wstride] of this
+// voxel: the plastic solve then starts from warm_x0 and, if that solve
+// fails, is redone from the plain start.
 template <int NS, class LoadF, class LoadS, class StoreS, class StoreP, class StoreT>
 CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int step, int iter,
                                   LoadF&& ldF, LoadS&& ldn, StoreS&& stn1, StoreP&& stP,
-                                  StoreT&& stT, Counters& cnt) {
+                                  StoreT&& stT, Counters& cnt, const float* ws = nullptr,
+                                  long wstride = 0) {
   double uddt[6], rn1[3][3];
   {
     double Fn[3][3], Fn1[3][3], rnh[3][3], fnhinv[3][3];
@@ -1029,6 +1195,23 @@ CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int s
     for (int k = 0; k < 6; ++k) { x[k] = n.stress[k]; ost[k] = x[k]; }
     double frac = 0.0, stp = 1.0;
     int cuts = 0, fail = 0;
+    if (ws) {
+      double x0[7];
+      warm_x0<NS>(P, o, pt, n, ws, wstride, x0);
+      x[6] = n.tt;
+      fail = local_solve<NS>(P, o, pt, n, P.dt, x, xprev, &ttrate1, cnt, x0);
+      bool ok = !fail && (x[6] == x[6]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ok &= (x[k] == x[k]);
+      if (ok) {
+        frac = 1.0;
+      } else {
+        cnt.ws_fallback++;
+        fail = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) x[k] = ost[k];
+      }
+    }
     while (frac < 1.0) {
       if (frac != 0.0 || stp != 1.0) {
         double Dw[6];

@@ -53,6 +53,15 @@ module cpfft_bindings
        integer(c_int) :: rc
      end function cpfft_tangent_homo
 
+     ! warm-start training pairs (no counterpart in the reference)
+     function cpfft_mlp_log(ctx, step, stride) bind(C, name="cpfft_mlp_log") result(rc)
+       import :: c_ptr, c_int, c_long
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: step
+       integer(c_long), value :: stride
+       integer(c_int) :: rc
+     end function cpfft_mlp_log
+
      ! update (update.f:16) + FFT_nr3.f:173-175
      function cpfft_update(ctx) bind(C, name="cpfft_update") result(rc)
        import :: c_ptr, c_int

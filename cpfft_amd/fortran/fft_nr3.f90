@@ -97,10 +97,14 @@ contains
   ! counters(:,step) = Newton iterations, CG iterations, mixed-BC loops,
   ! drive_eps_sig calls; times = pcg, sig-eps, total wall seconds (thyme
   ! categories 1 and 2, thyme.f:15-47).
+  ! Steps up to mlp_log_last log warm-start training pairs (every
+  ! mlp_log_stride-th voxel) of their converged update before the roll-over;
+  ! this has no counterpart in the reference and 0 turns it off.
   integer(c_int) function cpfft_compute(ctx, first_step, nstep, BC_all, isNBC_c, tolPCG, tolNR, &
-       maxIter, verbose, Pbar_out, counters, times) bind(C, name="cpfft_compute")
+       maxIter, verbose, Pbar_out, counters, times, mlp_log_last, mlp_log_stride) &
+       bind(C, name="cpfft_compute")
     type(c_ptr), value     :: ctx
-    integer(c_int), value  :: first_step, nstep, maxIter, verbose
+    integer(c_int), value  :: first_step, nstep, maxIter, verbose, mlp_log_last, mlp_log_stride
     real(c_double), value  :: tolPCG, tolNR
     real(c_double)         :: BC_all(9, nstep), Pbar_out(9, nstep), times(3)
     integer(c_int)         :: isNBC_c(9), counters(4, nstep)
@@ -273,6 +277,14 @@ contains
         barF = barF + DbarF
         iiter_NBC = iiter_NBC + 1
       end do
+
+      if (step <= mlp_log_last) then
+        rc = cpfft_mlp_log(ctx, step, int(max(mlp_log_stride, 1), c_long))
+        if (rc < 0) then
+          cpfft_compute = rc
+          return
+        end if
+      end if
 
       ! roll-over (FFT_nr3.f:173-179)
       barF_t(1:9) = barF(1:9)

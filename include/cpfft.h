@@ -150,6 +150,47 @@ int cpfft_counters(cpfft_ctx* ctx, long long out[8]);
 int cpfft_kernel_times(cpfft_ctx* ctx, double out[8]);
 int cpfft_set_timing(cpfft_ctx* ctx, int on);
 
+/* ---- learned warm start ------------------------------------------------
+ * No counterpart in the reference.  A 42 -> 128 -> 128 -> 12 tanh network,
+ * trained and evaluated with BF16 matrix-core GEMMs (FP32 accumulation),
+ * predicts each voxel's slip increments from its trial state.  The
+ * prediction only sets the starting point of the local Newton solve
+ * (mm10_a.f:2945-3023): the FP64 iteration and its tolerances are
+ * unchanged, and a warm-started solve that fails is redone from the plain
+ * start.  FCC (12 slip systems) only.
+ * Inputs (42): trial stress s_n + C:D in the lattice frame (3x3), the slip
+ * resistance once per system (12), the lattice-frame strain increment
+ * (3x3), the previous slip increments (12).  Outputs: slip increments.   */
+#define CPFFT_MLP_IN 42
+#define CPFFT_MLP_HID 128
+#define CPFFT_MLP_OUT 12
+#define CPFFT_MLP_NPARAM 23564 /* W1 (128x42) b1 W2 (128x128) b2 W3 (12x128) b3, rows = outputs */
+#define CPFFT_MLP_NNORM 108    /* mean_x[42] sd_x[42] mean_t[12] sd_t[12]                    */
+/* allocate the network (Glorot-uniform weights from seed), a log of
+ * `capacity` training pairs and work buffers for batches of `batch` rows  */
+int cpfft_mlp_init(cpfft_ctx* ctx, uint64_t seed, long capacity, int batch);
+/* append (inputs, converged slip increments) of every stride-th voxel of
+ * the current update (FN, FN1, STATE_N -> STATE_NP1) to the log           */
+int cpfft_mlp_log(cpfft_ctx* ctx, int step, long stride);
+/* normalise the log, then `iters` Adam steps (learning rate lr) on batches
+ * drawn with replacement; *loss = mean squared error (normalised units) of
+ * the last batch                                                          */
+int cpfft_mlp_train(cpfft_ctx* ctx, int iters, double lr, uint64_t seed, double* loss);
+/* on != 0: drive_eps_sig calls with iiter > 0 warm-start every voxel      */
+int cpfft_mlp_warm_start(cpfft_ctx* ctx, int on);
+/* parameters then normalisation (CPFFT_MLP_NPARAM + CPFFT_MLP_NNORM
+ * floats): get != 0 copies them out, get == 0 sets them                   */
+int cpfft_mlp_params(cpfft_ctx* ctx, float* buf, int get);
+/* network output Y[n][12] for normalised inputs Xn[n][42] (n <= batch)    */
+int cpfft_mlp_forward(cpfft_ctx* ctx, const float* Xn, int n, float* Y);
+/* gradient (CPFFT_MLP_NPARAM floats) and value of mean((Y - Tn)^2) over a
+ * batch of normalised inputs and targets (n <= batch)                     */
+int cpfft_mlp_grad(cpfft_ctx* ctx, const float* Xn, const float* Tn, int n, float* grad,
+                   double* loss);
+/* [0] logged pairs, [1] training steps, [2] warm-started voxel updates,
+ * [3] warm-started solves redone from the plain start                     */
+int cpfft_mlp_stats(cpfft_ctx* ctx, long long out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,25 +11,26 @@ using namespace cpfft;
 template <int NS>
 static void run(const cpfft_material& m, double dt, long V, const double* ang, int step,
                 int iter, const double* Fn, const double* Fn1, const double* sn, double* snp1,
-                double* Pn1, double* K4, long long* cnt) {
+                double* Pn1, double* K4, long long* cnt, const float* ws = nullptr) {
   double b[24][3], nn[24][3], st[6][6];
   int ns = 0;
   slip_systems(m.slip_set, ns, b, nn);
   cubic_stiffness(m, st);
   const MatParams P = make_params(m, dt);
-  long long tot[5] = {0, 0, 0, 0, 0};
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : tot[:5])
+  long long tot[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : tot[:6])
   for (long e = 0; e < V; ++e) {
     Orient<NS> o;
     const double a[3] = {ang[e], ang[V + e], ang[2 * V + e]};
     build_orient<NS>(st, b, nn, a, o);
-    Counters c{0, 0, 0, 0, 0};
+    Counters c{0, 0, 0, 0, 0, 0};
     double tan6[36];
     auto ldF = [&](int which, int k) { return which ? Fn1[k * V + e] : Fn[k * V + e]; };
     voxel_update<NS>(
         P, o, step, iter, ldF, [&](int w) { return sn[w * V + e]; },
         [&](int w, double v) { snp1[w * V + e] = v; },
-        [&](int k, double v) { Pn1[k * V + e] = v; }, [&](int k, double v) { tan6[k] = v; }, c);
+        [&](int k, double v) { Pn1[k * V + e] = v; }, [&](int k, double v) { tan6[k] = v; }, c,
+        ws ? ws + e : nullptr, V);
     double t6[6];
     for (int k = 0; k < 6; ++k) t6[k] = snp1[k * V + e];
     voxel_dPdF(ldF, t6, [&](int k) { return tan6[k]; },
@@ -39,8 +40,9 @@ static void run(const cpfft_material& m, double dt, long V, const double* ang, i
     tot[2] += c.cuts;
     tot[3] += c.failed;
     tot[4] += c.ls_evals;
+    tot[5] += c.ws_fallback;
   }
-  for (int k = 0; k < 5; ++k) cnt[k] = tot[k];
+  for (int k = 0; k < 6; ++k) cnt[k] = tot[k];
 }
 
 extern "C" int cpfft_host_drive_eps_sig(const cpfft_material* m, double dt, long V,
@@ -51,5 +53,16 @@ extern "C" int cpfft_host_drive_eps_sig(const cpfft_material* m, double dt, long
     run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt);
   else
     run<24>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt);
+  return 0;
+}
+
+// the same update warm-started from slip increments ws[12][V] (fcc)
+extern "C" int cpfft_host_drive_eps_sig_ws(const cpfft_material* m, double dt, long V,
+                                           const double* angles, int step, int iter,
+                                           const double* Fn, const double* Fn1, const double* sn,
+                                           double* snp1, double* Pn1, double* K4, long long* cnt,
+                                           const float* ws) {
+  if (m->slip_set != 1) return -1;
+  run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt, ws);
   return 0;
 }

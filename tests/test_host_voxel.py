@@ -43,7 +43,7 @@ def host_drive(mat, dt, ang, step, it, Fn, Fn1, sn):
     snp1 = np.zeros_like(sn)
     Pn1 = np.zeros((9, V))
     K4 = np.zeros((81, V))
-    cnt = (ctypes.c_longlong * 5)()
+    cnt = (ctypes.c_longlong * 6)()
     L.cpfft_host_drive_eps_sig(ctypes.byref(cm), dt, V, P(np.ascontiguousarray(ang)), step, it,
                                P(np.ascontiguousarray(Fn)), P(np.ascontiguousarray(Fn1)),
                                P(np.ascontiguousarray(sn)), P(snp1), P(Pn1), P(K4), cnt)
@@ -99,3 +99,43 @@ def test_host_update_matches_oracle(slip_set, jac_exact):
         o = O.drive_eps_sig(oc, ang, 2, it, dt, Fn, Fn1, sn)
         check(h, o, oc.nslip, (2, it))
     assert h[3][1] > 0  # plastic Newton iterations happened
+
+
+def test_warm_start_from_exact_slips_cuts_local_iterations():
+    """The warm-start mechanism (mm10_core.h warm_x0 and the predictor's
+    choice of start) with a perfect prediction: started from the converged
+    slip increments of the same update, the local solves converge to the
+    same state (the Newton tolerances are unchanged) in far fewer
+    iterations than from the reference's start."""
+    L = host_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.cpfft_host_drive_eps_sig_ws.argtypes = L.cpfft_host_drive_eps_sig.argtypes + [
+        ctypes.POINTER(ctypes.c_float)]
+    V = 6 ** 3
+    rng = np.random.default_rng(3)
+    _, ang = polycrystal(6, 5, 12)
+    mat = C.Material(jac_exact=1)
+    oc = O.crystal(jac_exact=1)
+    dt = 0.25
+    Fn = near_identity(rng, V, 0.0)
+    Fn1 = near_identity(rng, V, 1e-4, uniaxial=6e-4)
+    _, _, s1, _ = host_drive(mat, dt, ang, 1, 1, Fn, Fn1, O.state_init(oc, V))
+    Fn, Fn1 = Fn1, Fn1 + near_identity(rng, V, 2e-5, uniaxial=2.5e-4) - near_identity(rng, V, 0.0)
+    Pp, Kp, sp, cp = host_drive(mat, dt, ang, 2, 1, Fn, Fn1, s1)
+    g = sp[23:35]
+    u = np.sign(g) * (np.abs(g) / (mat.gamma_bar * dt)) ** (1.0 / mat.rate_n)
+    ws = np.ascontiguousarray(u.astype(np.float32))
+    cm = C.CMaterial(**{k: getattr(mat, k) for k, _ in C.CMaterial._fields_})
+    snp1 = np.zeros_like(s1)
+    Pn1 = np.zeros((9, V))
+    K4 = np.zeros((81, V))
+    cnt = (ctypes.c_longlong * 6)()
+    L.cpfft_host_drive_eps_sig_ws(ctypes.byref(cm), dt, V, P(np.ascontiguousarray(ang)), 2, 1,
+                                  P(Fn), P(np.ascontiguousarray(Fn1)), P(s1), P(snp1), P(Pn1),
+                                  P(K4), cnt, ws.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    plain = cp[0] + cp[1]
+    warm = cnt[0] + cnt[1]
+    assert cnt[5] == 0
+    assert warm < 0.5 * plain, (warm, plain)
+    assert rel(Pn1, Pp) < 1e-7
+    assert rel(snp1[0:6], sp[0:6]) < 1e-7
