@@ -65,17 +65,27 @@ class Workload:
     maxIter: int = 25
     gamma_mode: int = 1         # 1 textbook projection, 0 reference operator
     material: Material = field(default_factory=Material)
+    cycles: int = 0             # > 0: tension-compression cycles of amplitude `strain`
 
     def angles(self):
         return polycrystal(self.N, self.ngrains, self.seed)[1]
 
     def bc(self):
-        """(BC_all [nstep][9], isNBC [9]) for uniaxial tension along x"""
+        """(BC_all [nstep][9], isNBC [9]) for uniaxial loading along x:
+        monotonic tension to `strain`, or with cycles > 0 a triangle wave
+        0 -> +strain -> -strain -> 0 repeated `cycles` times (mults of
+        either sign, as a loading deck may give them)."""
         isNBC = np.zeros(9, np.int32)
         isNBC[[4, 8]] = 1
         FP_max = np.zeros(9)
         FP_max[0] = self.strain
-        mults = np.full(self.nstep, 1.0 / self.nstep)
+        if self.cycles > 0:
+            q = self.nstep // (4 * self.cycles)
+            assert q * 4 * self.cycles == self.nstep, "nstep must be a multiple of 4 * cycles"
+            quarter = np.full(q, 1.0 / q)
+            mults = np.tile(np.concatenate([quarter, -quarter, -quarter, quarter]), self.cycles)
+        else:
+            mults = np.full(self.nstep, 1.0 / self.nstep)
         BC = np.cumsum(FP_max[None, :] * mults[:, None], axis=0)
         for d in (0, 4, 8):
             if not isNBC[d]:
@@ -88,4 +98,8 @@ WORKLOADS = {
     "c1_16": Workload("c1_16", 16, 8, 20261202 + 1, 80, 0.02),
     "c2_64": Workload("c2_64", 64, 64, 20261202 + 2, 160, 0.04),
     "c3_128": Workload("c3_128", 128, 256, 20261202 + 3, 240, 0.06),
+    # BCC (the first 24 systems of bcc48), cyclic +-1.5%, 3 cycles; the cubic
+    # elastic constants of the FCC material are kept
+    "c5_256": Workload("c5_256", 256, 1024, 20261202 + 5, 720, 0.015, cycles=3,
+                       material=Material(slip_set=8)),
 }

@@ -159,15 +159,17 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   g.ty = 8;
   g.tz = 1;
   while (g.tz < 4 && 2 * 9 * (long)(2 * g.tz) * N * (long)sizeof(cplx) <= maxlds) g.tz *= 2;
-  if (2 * 9 * (long)N * (long)sizeof(cplx) > maxlds) {
-    cpfft_destroy(c);
-    return fail(-3, "cpfft_create: grid edge too large for the LDS of this device");
-  }
-  for (int s = 0; s < g.plan.nst; ++s)
-    if (g.plan.rad[s] > 16) {
+  if (!gkdf_pencil_supported(N)) {  // limits of the generic runtime-plan path
+    if (2 * 9 * (long)N * (long)sizeof(cplx) > maxlds) {
       cpfft_destroy(c);
-      return fail(-3, "cpfft_create: N has a prime factor above 16");
+      return fail(-3, "cpfft_create: grid edge too large for the LDS of this device");
     }
+    for (int s = 0; s < g.plan.nst; ++s)
+      if (g.plan.rad[s] > 16) {
+        cpfft_destroy(c);
+        return fail(-3, "cpfft_create: N has a prime factor above 16");
+      }
+  }
   // material
   double b[24][3], nn[24][3];
   slip_systems(m->slip_set, c->nslip, b, nn);

@@ -50,11 +50,22 @@ def kocks_angles(g):
     return np.degrees(np.stack([psi, th, phi], -1))
 
 
-def voronoi(rng, N, ngrains):
+def voronoi(rng, N, ngrains, method="auto"):
     """Grain id per voxel: nearest of ngrains seeds uniform in [0,1)^3 under
-    the periodic (minimum image) metric, voxel centres ((i+1/2)/N, ...)."""
+    the periodic (minimum image) metric, voxel centres ((i+1/2)/N, ...).
+    method: "brute" (every seed), "kdtree" (periodic k-d tree) or "auto"
+    (the tree for large grids)."""
     seeds = rng.random((ngrains, 3))
     c = (np.arange(N) + 0.5) / N
+    if method == "kdtree" or (method == "auto" and N ** 3 * ngrains > 5e9):
+        # large grids: periodic k-d tree (same nearest seed up to exact ties)
+        from scipy.spatial import cKDTree
+        tree = cKDTree(seeds, boxsize=1.0)
+        gid = np.empty(N ** 3, np.int32)
+        for ix in range(N):
+            X = np.stack(np.meshgrid([c[ix]], c, c, indexing="ij"), -1).reshape(-1, 3)
+            gid[ix * N * N:(ix + 1) * N * N] = tree.query(X, k=1)[1]
+        return gid
     gid = np.empty(N ** 3, np.int32)
     best = np.full(N ** 3, np.inf)
     X = np.stack(np.meshgrid(c, c, c, indexing="ij"), -1).reshape(-1, 3)

@@ -217,3 +217,10 @@ def test_P_is_first_piola_of_cauchy():
     sig = R @ T @ R.T
     Pe = np.linalg.det(F1) * sig @ np.linalg.inv(F1).T
     assert np.allclose(P.reshape(3, 3), Pe, rtol=1e-7, atol=1e-8 * abs(Pe).max())
+
+
+def test_voronoi_kdtree_matches_brute_force():
+    from cpfft_amd.microstructure import voronoi
+    a = voronoi(np.random.default_rng(4), 24, 100, method="brute")
+    b = voronoi(np.random.default_rng(4), 24, 100, method="kdtree")
+    assert np.array_equal(a, b)
