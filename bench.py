@@ -33,20 +33,23 @@ import cpfft_amd as C  # noqa: E402
 from cpfft_amd.configs import WORKLOADS  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per voxel of one G_K_dF with K4: read dF (72) + K4 (648),
-# write the result (72)
-GKDF_BYTES_PER_VOXEL = 792.0
+# The dominant kernel group is the CG operator of one iteration on the
+# compile-time-planned path: pass A forms the direction p = r + beta p and
+# K4:p, the five passes apply the projection, pass D writes q and the p.q
+# partial sums.  Algorithmic bytes per voxel: read p, r (72 each) and K4
+# (648), write p and q (72 each).
+CG_OPERATOR_BYTES_PER_VOXEL = 72.0 + 72.0 + 648.0 + 72.0 + 72.0
 
 
-def gkdf_traffic_per_voxel(N):
-    """HBM bytes per voxel that the five-pass G_K_dF with K4 moves: dF and K4
-    in, the result out, and eight passes over the nine-component half
-    spectrum (written by the z pass, read and written by the y, x and inverse
-    y passes, read by the inverse z pass).  At 128^3 this is 1377 B/voxel,
-    which FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel)
+def cg_operator_traffic_per_voxel(N):
+    """HBM bytes per voxel the five passes move for that group: pass A reads
+    r, p and K4 and writes p and the half spectrum; eight passes over the
+    nine-component half spectrum; pass D reads p again and writes q.  For
+    the operator alone (no CG fusion) the same count gives 1377 B/voxel at
+    128^3, which FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel)
     reproduce to 0.1%."""
     nz = N // 2 + 1
-    return 72.0 + 648.0 + 72.0 + 8.0 * 144.0 * nz / N
+    return 72.0 * 3 + 648.0 + 72.0 * 2 + 8.0 * 144.0 * nz / N
 
 
 def device_sync():
@@ -194,7 +197,7 @@ def main():
     n_gkdf = d[0]
     steps_per_s = K / wall
     gk_ms = avg[0]
-    achieved = GKDF_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
+    achieved = CG_OPERATOR_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
     out = {
         "metric": "converged load steps/s at 128^3 (plain method; warm start in warm_start)",
         "value": steps_per_s,
@@ -222,17 +225,18 @@ def main():
             "local_failures": d[6],
         },
         "category_s": {"pcg": float(times[0]), "sig_eps": float(times[1]), "total": float(times[2])},
-        "kernel_ms": {"G_K_dF_with_K4": avg[0], "drive_eps_sig": avg[1], "cg_update": avg[2],
+        "kernel_ms": {"cg_operator": avg[0], "drive_eps_sig": avg[1], "cg_update": avg[2],
                       "G_K_dF_no_K4": avg[3]},
         "roofline": {
-            "kernel": "G_K_dF with K4 (5-kernel launch group)",
+            "kernel": ("CG operator: p = r + beta p, G_K_dF with K4, p.q "
+                       "(5-kernel launch group)"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": gkdf_traffic_per_voxel(wl.N) * V,
-            "algorithmic_bytes_per_launch": GKDF_BYTES_PER_VOXEL * V,
+            "traffic": cg_operator_traffic_per_voxel(wl.N) * V,
+            "algorithmic_bytes_per_launch": CG_OPERATOR_BYTES_PER_VOXEL * V,
             "avg_launch_ms": gk_ms,
             "launches_measured": int(nlaunch[0]),
         },

@@ -52,6 +52,7 @@ struct cpfft_ctx {
   double* tmp9 = nullptr;  // tangent_homo right-hand side
   double* tan6 = nullptr;  // per-voxel material tangent [36][V]
   double* partials = nullptr;
+  double* dpart = nullptr;  // per-block p.q partial sums of the fused pass D
   double* dres = nullptr;  // device results of reductions (16 doubles)
   double* hres = nullptr;  // pinned host mirror
   CGScalars* dcg = nullptr;  // device scalars of the running CG solve
@@ -99,7 +100,7 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < CPFFT_NFIELDS; ++f)
     if (c->fields[f]) hipFree(c->fields[f]);
-  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dres, c->orients, c->oid, c->dcnt, c->dcg};
+  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dpart, c->dres, c->orients, c->oid, c->dcnt, c->dcg};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
@@ -213,6 +214,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipMalloc(&c->tmp9, 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->tan6, 36 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->partials, reduce_partials_count(9 * V) * sizeof(double)));
+  HIP_OK(hipMalloc(&c->dpart, std::max(1, gkdf_dot_parts(N)) * sizeof(double)));
   HIP_OK(hipMalloc(&c->dres, 16 * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcg, sizeof(CGScalars)));
@@ -439,6 +441,9 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   HIP_OK(hipMemcpyAsync(c->dcg, c->hcg, sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
   SpecGeom g = c->geom;
   g.skip = &c->dcg->done;
+  const bool fused = gkdf_pencil_supported(c->N);
+  const int ndp = gkdf_dot_parts(c->N);
+  const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart};
   const int chunk = 8;
   int launched = 0;
   for (;;) {
@@ -446,12 +451,22 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
       flush_events(c);
     for (int j = 0; j < chunk; ++j) {
       tstart(c, 0);
-      launch_gkdf(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
+      if (fused) {
+        // pass A forms p = r + beta p, pass D the p.q partial sums
+        launch_gkdf_pencil(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream, &fz);
+      } else {
+        launch_gkdf(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
+      }
       tstop(c, 0);
       tstart(c, 2);
-      launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
+      if (fused) {
+        launch_cg_alpha(c->dpart, ndp, c->dcg, c->stream);
+        launch_cg_update_check(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
+      } else {
+        launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
+      }
       tstop(c, 2);
-      launch_cg_dir_dev(n, c->dcg, c->r, c->p, c->stream);
+      if (!fused) launch_cg_dir_dev(n, c->dcg, c->r, c->p, c->stream);
     }
     launched += chunk;
     HIP_OK(hipGetLastError());
@@ -566,16 +581,30 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
     pp.p[i] = Pd[i];
     pp.q[i] = Q[i];
   }
+  const bool fused = gkdf_pencil_supported(c->N);
+  const int ndp = gkdf_dot_parts(c->N);
+  Ptr9r rr;
+  for (int i = 0; i < 9; ++i) rr.r[i] = R[i];
   const int chunk = 8;
   for (;;) {
     for (int j = 0; j < chunk; ++j) {
-      launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream);
+      // K4 : p_i for every running solve; with the fused path the direction
+      // update p_i = r_i + beta_i p_i happens in the same pass
+      launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream, fused ? &rr : nullptr);
       for (int i = 0; i < 9; ++i) {
         SpecGeom g = c->geom;
         g.skip = &c->dcg9[i].done;
-        launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);  // in place: pass A reads all of Q[i] first
-        launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
-        launch_cg_dir_dev(n, c->dcg9 + i, R[i], Pd[i], c->stream);
+        // in place: pass A reads all of Q[i] before pass D writes it
+        if (fused) {
+          const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart};
+          launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
+          launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream);
+          launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
+        } else {
+          launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);
+          launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
+          launch_cg_dir_dev(n, c->dcg9 + i, R[i], Pd[i], c->stream);
+        }
       }
     }
     HIP_OK(hipGetLastError());

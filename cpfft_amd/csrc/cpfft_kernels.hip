@@ -530,6 +530,15 @@ __global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const doubl
     p[i] = r[i] + b * p[i];
 }
 
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s) {
+  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs);
+}
+void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
+                            double* r, double* partials, hipStream_t s) {
+  k_cg_update_s<<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
+  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+}
+
 void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,
                     double* r, double* partials, hipStream_t s) {
   k_dot_part_s<<<kRedBlocks, kBlock, 0, s>>>(p, q, n, partials, cs);
@@ -544,9 +553,10 @@ void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, 
 // ------------------------------------------ multi-RHS K4 contraction
 // q_i = K4 : p_i for the right-hand sides i < nrhs whose solve is still
 // running (tangent_homo's nine solves share K4, which is read once).
+template <bool DIR>
 __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const double* __restrict__ K4,
                                                               Ptr9 pp, const CGScalars* __restrict__ cs,
-                                                              int nrhs) {
+                                                              int nrhs, Ptr9r rr) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= V) return;
   bool any = false;
@@ -558,16 +568,30 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
   for (int i = 0; i < nrhs; ++i) {
     if (cs[i].done) continue;
     double f[9];
+    if (DIR) {
+      const double beta = cs[i].beta;
+      double* pw = const_cast<double*>(pp.p[i]);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) f[c] = pp.p[i][c * V + e];
+      for (int c = 0; c < 9; ++c) {
+        f[c] = rr.r[i][c * V + e] + beta * pw[c * V + e];
+        pw[c * V + e] = f[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) f[c] = pp.p[i][c * V + e];
+    }
 #pragma unroll
     for (int r = 0; r < 9; ++r) pp.q[i][r * V + e] = ddot42n_row(k + 9 * r, f);
   }
 }
 
 void launch_k4_contract_multi(long V, const double* K4, const Ptr9& pp, const CGScalars* cs,
-                              int nrhs, hipStream_t s) {
-  k_k4_contract_multi<<<(unsigned)((V + kBlock - 1) / kBlock), kBlock, 0, s>>>(V, K4, pp, cs, nrhs);
+                              int nrhs, hipStream_t s, const Ptr9r* rr) {
+  const unsigned g = (unsigned)((V + kBlock - 1) / kBlock);
+  if (rr)
+    k_k4_contract_multi<true><<<g, kBlock, 0, s>>>(V, K4, pp, cs, nrhs, *rr);
+  else
+    k_k4_contract_multi<false><<<g, kBlock, 0, s>>>(V, K4, pp, cs, nrhs, Ptr9r{});
 }
 
 // ------------------------------------------------------- stress update

@@ -205,12 +205,17 @@ constexpr int threads_z(int n, bool hask) { return hask && tile_z(n) * n / 2 >= 
 // components are transformed one after the other; the K4 rows (or F
 // component) of component c+1 are loaded into a second register set before
 // component c is transformed, so the loads overlap the LDS work.
-template <int N, bool HASK>
-__global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* __restrict__ F,
+// DIR: F is the CG direction p, first replaced by p = R + cs->beta p (each
+// element read and written by the same thread).
+template <int N, bool HASK, bool DIR>
+__global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* F,
                                                               const double* __restrict__ K4,
                                                               cplx* __restrict__ T1,
                                                               const cplx* __restrict__ twg,
-                                                              const int* __restrict__ skip) {
+                                                              const int* __restrict__ skip,
+                                                              double* Pd,
+                                                              const double* __restrict__ R,
+                                                              const CGScalars* __restrict__ cs) {
   if (skip && *skip) return;
   constexpr int TA = threads_z(N, HASK);
   constexpr int TY = tile_z(N), P = TY * N, PT = (P + TA - 1) / TA, LP = TY / 2;
@@ -225,12 +230,21 @@ __global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* __re
   auto inb = [&](int p) { return P % TA == 0 || p < P; };
   double f[HASK ? PT : 1][9];
   if constexpr (HASK) {
+    const double beta = DIR ? cs->beta : 0.0;
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
       const int p = threadIdx.x + u * TA;
       if (inb(p)) {
 #pragma unroll
-        for (int d = 0; d < 9; ++d) f[u][d] = F[d * V + e0 + p];
+        for (int d = 0; d < 9; ++d) {
+          if constexpr (DIR) {
+            const double v = R[d * V + e0 + p] + beta * Pd[d * V + e0 + p];
+            Pd[d * V + e0 + p] = v;
+            f[u][d] = v;
+          } else {
+            f[u][d] = F[d * V + e0 + p];
+          }
+        }
       }
     }
   }
@@ -369,10 +383,13 @@ __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __
 
 // ---- pass D: complex-to-real FFT along z, two pencils per FFT --------------
 // grid (N/TY, N, 9): blockIdx.x = y tile, blockIdx.y = x, blockIdx.z = c
-template <int N>
+// DOT: also writes the block's partial sum of Pd . out to part[block].
+template <int N, bool DOT>
 __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, double* __restrict__ out,
                                               const cplx* __restrict__ twg,
-                                              const int* __restrict__ skip) {
+                                              const int* __restrict__ skip,
+                                              const double* __restrict__ Pd,
+                                              double* __restrict__ part) {
   if (skip && *skip) return;
   constexpr int TY = tile_z(N), P = TY * N, LP = TY / 2, LS = lstride(N), NZ = N / 2 + 1;
   constexpr long V = (long)N * N * N;
@@ -393,28 +410,53 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
   fft<N, LP, false>(buf, tw);
   const double sc = 1.0 / (double)V;
   const long e0 = ((long)x * N + y0) * N;
+  double acc = 0.0;
   for (int p = threadIdx.x; p < P; p += kT) {
     const int yy = p / N, z = p - yy * N;
     const cplx v = buf[(yy >> 1) * LS + pidx(z)];
-    out[c * V + e0 + p] = ((yy & 1) ? v.im : v.re) * sc;
+    const double q = ((yy & 1) ? v.im : v.re) * sc;
+    out[c * V + e0 + p] = q;
+    if constexpr (DOT) acc += q * Pd[c * V + e0 + p];
+  }
+  if constexpr (DOT) {
+    __shared__ double red[kT];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kT / 2; h > 0; h >>= 1) {
+      if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = red[0];
   }
 }
 
 template <int N>
 void launch(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* T1, cplx* T2,
-            const cplx* tw, hipStream_t s) {
+            const cplx* tw, hipStream_t s, const CGFuse* fz) {
   constexpr int TY = tile_z(N), TX = tile_y(N), NZ = N / 2 + 1;
   static_assert(N % TY == 0 && TY % 2 == 0 && N % TX == 0, "tile shapes");
   const int* sk = g.skip;
-  if (K4)
-    kA_zfwd<N, true><<<dim3(N / TY, N), threads_z(N, true), 0, s>>>(F, K4, T1, tw, sk);
+  const dim3 ga(N / TY, N);
+  if (K4 && fz && fz->dir_p)
+    kA_zfwd<N, true, true><<<ga, threads_z(N, true), 0, s>>>(F, K4, T1, tw, sk, fz->dir_p, fz->dir_r,
+                                                             fz->cs);
+  else if (K4)
+    kA_zfwd<N, true, false><<<ga, threads_z(N, true), 0, s>>>(F, K4, T1, tw, sk, nullptr, nullptr,
+                                                              nullptr);
   else
-    kA_zfwd<N, false><<<dim3(N / TY, N), threads_z(N, false), 0, s>>>(F, nullptr, T1, tw, sk);
+    kA_zfwd<N, false, false><<<ga, threads_z(N, false), 0, s>>>(F, nullptr, T1, tw, sk, nullptr,
+                                                                nullptr, nullptr);
   kB_y<N, true><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T1, T2, tw, sk);
   kC_x<N><<<dim3(N / ylines_x(N), NZ), kT, 0, s>>>(T2, tw, g.gamma_mode, sk);
   kB_y<N, false><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T2, T1, tw, sk);
-  kD_zinv<N><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk);
+  if (fz && fz->dot_p)
+    kD_zinv<N, true><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, fz->dot_p, fz->dot_part);
+  else
+    kD_zinv<N, false><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, nullptr, nullptr);
 }
+
+template <int N>
+constexpr int dot_parts() { return (N / tile_z(N)) * N * 9; }
 
 }  // namespace fp
 
@@ -426,18 +468,31 @@ bool gkdf_pencil_supported(int N) {
 }
 
 bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, double* out,
-                        cplx* spec, const cplx* tw, hipStream_t s) {
+                        cplx* spec, const cplx* tw, hipStream_t s, const CGFuse* fuse) {
   cplx* T1 = spec;
   cplx* T2 = spec + 9 * g.S1;
   switch (g.N) {
-    case 16: fp::launch<16>(g, F, K4, out, T1, T2, tw, s); return true;
-    case 32: fp::launch<32>(g, F, K4, out, T1, T2, tw, s); return true;
-    case 48: fp::launch<48>(g, F, K4, out, T1, T2, tw, s); return true;
-    case 64: fp::launch<64>(g, F, K4, out, T1, T2, tw, s); return true;
-    case 96: fp::launch<96>(g, F, K4, out, T1, T2, tw, s); return true;
-    case 128: fp::launch<128>(g, F, K4, out, T1, T2, tw, s); return true;
-    case 256: fp::launch<256>(g, F, K4, out, T1, T2, tw, s); return true;
+    case 16: fp::launch<16>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    case 32: fp::launch<32>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    case 48: fp::launch<48>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    case 64: fp::launch<64>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    case 96: fp::launch<96>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    case 128: fp::launch<128>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    case 256: fp::launch<256>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
     default: return false;
+  }
+}
+
+int gkdf_dot_parts(int N) {
+  switch (N) {
+    case 16: return fp::dot_parts<16>();
+    case 32: return fp::dot_parts<32>();
+    case 48: return fp::dot_parts<48>();
+    case 64: return fp::dot_parts<64>();
+    case 96: return fp::dot_parts<96>();
+    case 128: return fp::dot_parts<128>();
+    case 256: return fp::dot_parts<256>();
+    default: return 0;
   }
 }
 

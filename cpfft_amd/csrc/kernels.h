@@ -38,12 +38,33 @@ struct CGScalars {
 // Gamma:(K4:F) or Gamma:F (K4 == nullptr).  spec: 2*9*S1 complex scratch.
 // tw: the N twiddles exp(-2 pi i k/N).  Uses the compile-time-planned path
 // of fft_pencil.hip where it exists for N, else the generic runtime plan.
+// CG work fused into the operator's passes (compile-time-planned path only):
+// dir_p != nullptr: pass A first sets dir_p = dir_r + cs->beta * dir_p (the
+//   CG direction, F must be dir_p) and transforms the new direction;
+// dot_p != nullptr: pass D writes per-block partial sums of dot_p . out
+//   into dot_part (gkdf_dot_parts(N) values, fixed order).
+struct CGFuse {
+  double* dir_p;
+  const double* dir_r;
+  const struct CGScalars* cs;
+  const double* dot_p;
+  double* dot_part;
+};
+
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
                  const cplx* tw, hipStream_t s);
 bool gkdf_pencil_supported(int N);
 // returns false (and launches nothing) when N has no compile-time plan
 bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, double* out,
-                        cplx* spec, const cplx* tw, hipStream_t s);
+                        cplx* spec, const cplx* tw, hipStream_t s, const CGFuse* fuse = nullptr);
+// number of partial sums pass D writes for a fused dot product
+int gkdf_dot_parts(int N);
+
+// alpha = rr / sum(part[0..np)) (the p.q partial sums of a fused pass D)
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s);
+// x += alpha p; r -= alpha q; convergence test (no p.q, no direction update)
+void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
+                            double* r, double* partials, hipStream_t s);
 
 // One CG iteration after q = A p: alpha = rr / (p.q); x += alpha p;
 // r -= alpha q; convergence test on |r|; p = r + beta p.  All scalars stay
@@ -58,9 +79,13 @@ struct Ptr9 {
   const double* p[9];
   double* q[9];
 };
-// q_i = K4 : p_i (ddot42n) for every i < nrhs with cs[i].done == 0
+// q_i = K4 : p_i (ddot42n) for every i < nrhs with cs[i].done == 0; with
+// r != nullptr the direction is first updated, p_i = r_i + cs[i].beta p_i
+struct Ptr9r {
+  const double* r[9];
+};
 void launch_k4_contract_multi(long V, const double* K4, const Ptr9& pp, const CGScalars* cs,
-                              int nrhs, hipStream_t s);
+                              int nrhs, hipStream_t s, const Ptr9r* rr = nullptr);
 
 // reductions: result -> d_out[0..nres) on device (two-stage, fixed order)
 void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,

@@ -9,9 +9,13 @@ Operators (the reference routines they replace):
   drive_eps_sig      drive_eps_sig.f:16-45 (Newton iteration 1 of step 1)
   dcopy              a plain copy of one [9][V] field, the bandwidth yardstick
                      for hardware-counter calibration
+  fftPcg iteration   FFT_nr3.f:214-360 (with --cg): the CG operator, which
+                     also forms p = r + beta p and the p.q partial sums, and
+                     the alpha / x, r update / residual-check kernels
 
 Prints one JSON line with the average ms and the algorithmic HBM GB/s of each
-(bytes per voxel: 792 with K4, 144 without, 144 for the copy).
+(bytes per voxel: 792 with K4, 144 without, 144 for the copy, 936 for the
+CG operator: read p, r, K4, write p, q).
 """
 import argparse
 import json
@@ -25,6 +29,7 @@ from .configs import WORKLOADS
 BYTES_GKDF_K4 = 792.0   # read dF 72 + K4 648, write 72
 BYTES_GKDF = 144.0      # read 72, write 72
 BYTES_COPY = 144.0
+BYTES_CG_OP = 936.0     # read p 72, r 72, K4 648, write p 72, q 72
 
 
 def main():
@@ -32,6 +37,7 @@ def main():
     ap.add_argument("--workload", default="c3_128")
     ap.add_argument("--reps", type=int, default=40)
     ap.add_argument("--skip-material", action="store_true")
+    ap.add_argument("--cg", action="store_true", help="also time one CG solve")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     V = wl.N ** 3
@@ -69,6 +75,17 @@ def main():
         "dcopy_ms_wall": cp, "dcopy_GBs_wall": BYTES_COPY * V / cp * 1e-6,
         "gkdf_wall_s": wall,
     }
+    if args.cg:
+        # right-hand side K4:dF projected; the solve runs to 1e-8 relative
+        s.gkdf(DFM, B, 1)
+        s.sync()
+        s.set_timing(True)
+        rc, it, rel = s.pcg(B, WORK, 1e-8)
+        s.sync()
+        avg, n = s.kernel_times()
+        res.update({"cg_rc": rc, "cg_iters": it, "cg_relres": rel,
+                    "cg_operator_ms": avg[0], "cg_operator_GBs": BYTES_CG_OP * V / avg[0] * 1e-6,
+                    "cg_update_ms": avg[2], "cg_iter_ms": avg[0] + avg[2]})
     if not args.skip_material:
         s.upload(FN1, np.tile(np.array([1.0005, 0, 0, 0, 0.99985, 0, 0, 0, 0.99985])[:, None],
                               (1, V)))
