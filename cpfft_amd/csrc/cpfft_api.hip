@@ -442,6 +442,7 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   SpecGeom g = c->geom;
   g.skip = &c->dcg->done;
   const bool fused = gkdf_pencil_supported(c->N);
+  // pass A forms p = r + beta p, pass D the p.q partial sums
   const int ndp = gkdf_dot_parts(c->N);
   const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart};
   const int chunk = 8;
@@ -452,7 +453,6 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
     for (int j = 0; j < chunk; ++j) {
       tstart(c, 0);
       if (fused) {
-        // pass A forms p = r + beta p, pass D the p.q partial sums
         launch_gkdf_pencil(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream, &fz);
       } else {
         launch_gkdf(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);

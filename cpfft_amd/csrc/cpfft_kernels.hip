@@ -475,11 +475,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ 
                                                      CGScalars* __restrict__ cs) {
   if (cs->done) return;
   __shared__ double sh[kBlock];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
-  double pq = block_sum(acc, sh);
+  const double pq = ordered_sum<kBlock>(part, np, sh);
   if (threadIdx.x == 0) cs->alpha = cs->rr / pq;
 }
+
 __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars* __restrict__ cs,
                                                         const double* __restrict__ p,
                                                         const double* __restrict__ q,
@@ -501,26 +500,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars*
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// |r| test (FFT_nr3.f:304-306): converged at |r| <= tol |b| or |r| <= tol
+// |r| test from the r.r partial sums (FFT_nr3.f:304-306)
 __global__ __launch_bounds__(kBlock) void k_cg_check(const double* __restrict__ part, int np,
                                                      CGScalars* __restrict__ cs) {
   if (cs->done) return;
   __shared__ double sh[kBlock];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
-  double rr1 = block_sum(acc, sh);
-  if (threadIdx.x == 0) {
-    cs->k += 1;
-    const double res = sqrt(rr1);
-    cs->resnorm = res;
-    if (res <= cs->tolb || res <= cs->tol) {
-      cs->done = 1;
-    } else {
-      cs->beta = rr1 / cs->rr;
-      cs->rr = rr1;
-      if (cs->k >= cs->maxit) cs->done = 2;
-    }
-  }
+  const double rr1 = ordered_sum<kBlock>(part, np, sh);
+  if (threadIdx.x == 0) cg_check(cs, rr1);
 }
 __global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const double* __restrict__ r,
                            double* __restrict__ p) {
@@ -530,13 +516,13 @@ __global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const doubl
     p[i] = r[i] + b * p[i];
 }
 
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s) {
-  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs);
-}
 void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
                             double* r, double* partials, hipStream_t s) {
   k_cg_update_s<<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
   k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+}
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s) {
+  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs);
 }
 
 void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,

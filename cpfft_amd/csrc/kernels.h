@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "cg_device.h"
 #include "cpfft_common.h"
 #include "green.h"
 #include "mm10_core.h"
@@ -22,19 +23,6 @@ struct SpecGeom {
   const int* skip;  // if non-null and *skip != 0, every block returns at once
 };
 
-// Scalars of one conjugate-gradient solve, kept in device memory so that
-// iterations run without host round trips (fftPcg, FFT_nr3.f:214-360).
-struct CGScalars {
-  double rr;       // r.r of the current residual
-  double alpha, beta;
-  double tolb, tol;
-  double resnorm;  // |r| after the last update
-  int k;           // iterations completed
-  int maxit;
-  int done;        // 0 running, 1 converged, 2 iteration limit reached
-  int pad;
-};
-
 // Gamma:(K4:F) or Gamma:F (K4 == nullptr).  spec: 2*9*S1 complex scratch.
 // tw: the N twiddles exp(-2 pi i k/N).  Uses the compile-time-planned path
 // of fft_pencil.hip where it exists for N, else the generic runtime plan.
@@ -46,7 +34,7 @@ struct CGScalars {
 struct CGFuse {
   double* dir_p;
   const double* dir_r;
-  const struct CGScalars* cs;
+  CGScalars* cs;
   const double* dot_p;
   double* dot_part;
 };
@@ -62,7 +50,8 @@ int gkdf_dot_parts(int N);
 
 // alpha = rr / sum(part[0..np)) (the p.q partial sums of a fused pass D)
 void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s);
-// x += alpha p; r -= alpha q; convergence test (no p.q, no direction update)
+// x += alpha p; r -= alpha q with the r.r partial sums, then the convergence
+// test and beta (no p.q, no direction update)
 void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
                             double* r, double* partials, hipStream_t s);
 

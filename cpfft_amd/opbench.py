@@ -12,6 +12,8 @@ Operators (the reference routines they replace):
   fftPcg iteration   FFT_nr3.f:214-360 (with --cg): the CG operator, which
                      also forms p = r + beta p and the p.q partial sums, and
                      the alpha / x, r update / residual-check kernels
+  tangent_homo       tangent_homo.f:11-73 (with --homo): the nine CG solves
+                     run together, host wall time of one call
 
 Prints one JSON line with the average ms and the algorithmic HBM GB/s of each
 (bytes per voxel: 792 with K4, 144 without, 144 for the copy, 936 for the
@@ -38,6 +40,7 @@ def main():
     ap.add_argument("--reps", type=int, default=40)
     ap.add_argument("--skip-material", action="store_true")
     ap.add_argument("--cg", action="store_true", help="also time one CG solve")
+    ap.add_argument("--homo", action="store_true", help="also time one tangent_homo call")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     V = wl.N ** 3
@@ -86,6 +89,14 @@ def main():
         res.update({"cg_rc": rc, "cg_iters": it, "cg_relres": rel,
                     "cg_operator_ms": avg[0], "cg_operator_GBs": BYTES_CG_OP * V / avg[0] * 1e-6,
                     "cg_update_ms": avg[2], "cg_iter_ms": avg[0] + avg[2]})
+    if args.homo:
+        c0 = s.counters()
+        t0 = time.perf_counter()
+        s.tangent_homo(1e-8)
+        s.sync()
+        res["homo_wall_s"] = time.perf_counter() - t0
+        res["homo_cg_iterations"] = int(s.counters()[1] - c0[1])
+        res["homo_ms_per_rhs_iteration"] = res["homo_wall_s"] * 1e3 / max(1, res["homo_cg_iterations"])
     if not args.skip_material:
         s.upload(FN1, np.tile(np.array([1.0005, 0, 0, 0, 0.99985, 0, 0, 0, 0.99985])[:, None],
                               (1, V)))
