@@ -11,6 +11,9 @@ steps, trains the network on the GPU, and times the same K steps
 warm-started; its rate and its averaged-stress difference from the plain run
 are reported beside the headline.
 
+With --workload c4_batch the run is the batched uneven-RVE workload
+instead (24 RVEs of edge 32-96 solved at once; see batch_bench).
+
 Prints one JSON line.  The roofline object describes G_K_dF with K4 (the
 CG matrix-vector product, five kernels launched as one group), whose device
 time is measured with HIP events on the solver's stream inside the timed
@@ -30,7 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import cpfft_amd as C  # noqa: E402
-from cpfft_amd.configs import WORKLOADS  # noqa: E402
+from cpfft_amd.configs import BATCH_WORKLOADS, WORKLOADS  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # The dominant kernel group is the CG operator of one iteration on the
@@ -161,6 +164,82 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
     }
 
 
+def batch_bench(name, W, K, nthreads):
+    """Batched uneven RVEs: the members' load steps W+1..W+K timed twice
+    from the same warm-up state, once one RVE after another on one host
+    thread with kernel-by-kernel launches, once through cpfft_compute_batch
+    on nthreads threads with graph-replayed CG chunks.  Returns the JSON
+    record (value: the batched aggregate rate)."""
+    bw = BATCH_WORKLOADS[name]
+    members = bw.members()
+    angles = [w.angles() for w in members]
+    bcs = [w.bc() for w in members]
+    vox = sum(w.N ** 3 for w in members)
+    print(f"[bench] {name}: {len(members)} RVEs, edges {[w.N for w in members]}, "
+          f"{vox} voxels in all, warmup={W} steps={K}", flush=True)
+    legs = {}
+    for leg, nt, graphs in (("sequential", 1, False), ("batched", nthreads, True)):
+        solvers = []
+        for w, ang in zip(members, angles):
+            sv = C.Solver(w.N, ang, w.material, dt=w.dt, gamma_mode=w.gamma_mode)
+            sv.set_graphs(graphs)
+            solvers.append(sv)
+
+        def jobs(lo, hi, first):
+            return [(sv, bc[0][lo:hi], bc[1], w.tolPCG, w.tolNR, w.maxIter, first)
+                    for sv, w, bc in zip(solvers, members, bcs)]
+        rc, _ = C.compute_batch(jobs(0, W, 1), nthreads=nt)
+        if rc != 0:
+            raise SystemExit(f"{leg} warm-up failed with code {rc}")
+        c0 = [sv.counters() for sv in solvers]
+        device_sync()
+        t0 = time.perf_counter()
+        rc, res = C.compute_batch(jobs(W, W + K, W + 1), nthreads=nt)
+        for sv in solvers:
+            sv.sync()
+        device_sync()
+        wall = time.perf_counter() - t0
+        if rc != 0:
+            raise SystemExit(f"{leg} timed steps failed with code {rc}")
+        c1 = [sv.counters() for sv in solvers]
+        legs[leg] = {
+            "wall_s": wall,
+            "voxel_load_steps_per_s": vox * K / wall,
+            "rve_load_steps_per_s": len(members) * K / wall,
+            "voxel_updates_per_s": sum(w.N ** 3 * (b[2] - a[2])
+                                       for w, a, b in zip(members, c0, c1)) / wall,
+            "cg_iterations": int(sum(b[1] - a[1] for a, b in zip(c0, c1))),
+            "threads": nt, "graphs": graphs,
+            "Pbar": [r[1] for r in res],
+            "newton": [int(r[2][:, 0].sum()) for r in res],
+        }
+        print(f"[bench] {leg}: {wall:.2f} s, {vox * K / wall:.3e} voxel load steps/s", flush=True)
+        for sv in solvers:
+            sv.close()
+    seq, bat = legs["sequential"], legs["batched"]
+    same = all(np.array_equal(a, b) for a, b in zip(seq.pop("Pbar"), bat.pop("Pbar")))
+    return {
+        "metric": f"aggregate voxel load steps/s over {len(members)} uneven RVEs",
+        "value": bat["voxel_load_steps_per_s"],
+        "unit": "voxel load steps/s",
+        "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": 1e3 * bat["wall_s"] / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": (f"{name}: {len(members)} periodic Voronoi RVEs, edges {sorted(set(bw.edges))}, "
+                         f"{bw.grains[0]}-{bw.grains[1]} grains, FCC cubic Voce n=20, F = I + eps n(x)n "
+                         f"along random n, {bw.dstrain} strain per step at rates "
+                         f"{bw.rates[0]}-{bw.rates[1]}/s, steps {W + 1}-{W + K}"),
+            "edges": [w.N for w in members], "grains": [w.ngrains for w in members],
+            "dt": [w.dt for w in members], "voxels": vox,
+        },
+        "batched": bat, "sequential": seq,
+        "speedup_vs_sequential": seq["wall_s"] / bat["wall_s"],
+        "Pbar_bitwise_equal_to_sequential": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,11 +248,16 @@ def main():
     ap.add_argument("--workload", default="c3_128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--threads", type=int, default=8, help="host threads of a batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+              "(the solver step is built and measured on one GPU's share)", file=sys.stderr)
         sys.exit(2)
+    if args.workload in BATCH_WORKLOADS:
+        print(json.dumps(batch_bench(args.workload, args.warmup, args.steps, args.threads)),
+              flush=True)
+        return
 
     wl = WORKLOADS[args.workload]
     W, K = args.warmup, args.steps

@@ -39,13 +39,29 @@ class CMaterial(ctypes.Structure):
 # every symbol include/cpfft.h declares
 EXPORTS = [
     "cpfft_create", "cpfft_destroy", "cpfft_last_error", "cpfft_nslip", "cpfft_field",
-    "cpfft_upload", "cpfft_download", "cpfft_reset", "cpfft_sync", "cpfft_gkdf", "cpfft_pcg",
+    "cpfft_upload", "cpfft_download", "cpfft_reset", "cpfft_sync", "cpfft_driver_state",
+    "cpfft_gkdf", "cpfft_pcg",
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
     "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_counters",
-    "cpfft_kernel_times", "cpfft_set_timing",
+    "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
     "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_stats",
 ]
+
+# every symbol include/cpfft_driver.h declares (libcpfft_driver.so)
+DRIVER_EXPORTS = ["cpfft_compute", "cpfft_compute_batch"]
+
+
+class CBatchItem(ctypes.Structure):
+    """cpfft_batch_item of include/cpfft_driver.h"""
+    _fields_ = [
+        ("ctx", _vp), ("first_step", ctypes.c_int), ("nstep", ctypes.c_int),
+        ("BC_all", _dp), ("isNBC", _ip), ("tolPCG", ctypes.c_double),
+        ("tolNR", ctypes.c_double), ("maxIter", ctypes.c_int), ("weight", ctypes.c_double),
+        ("Pbar_out", _dp), ("counters", _ip), ("times", ctypes.c_double * 3),
+        ("rc", ctypes.c_int),
+    ]
+
 
 MLP_IN, MLP_HID, MLP_OUT = 42, 128, 12
 MLP_NPARAM, MLP_NNORM = 23564, 108
@@ -69,6 +85,9 @@ def lib():
         L.cpfft_nslip.argtypes = [_vp]
         L.cpfft_field.argtypes = [_vp, ctypes.c_int]
         L.cpfft_field.restype = _vp
+        L.cpfft_set_graphs.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_driver_state.argtypes = [_vp]
+        L.cpfft_driver_state.restype = _dp
         L.cpfft_upload.argtypes = [_vp, ctypes.c_int, _dp]
         L.cpfft_download.argtypes = [_vp, ctypes.c_int, _dp]
         L.cpfft_reset.argtypes = [_vp]
@@ -111,6 +130,7 @@ def driver():
         D.cpfft_compute.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _ip, _dp,
                                     ctypes.c_int, ctypes.c_int]
+        D.cpfft_compute_batch.argtypes = [ctypes.POINTER(CBatchItem), ctypes.c_int, ctypes.c_int]
         _drv = D
     return _drv
 
@@ -237,6 +257,10 @@ class Solver:
         _check(lib().cpfft_counters(self.h, out), "cpfft_counters")
         return list(out)
 
+    def set_graphs(self, on):
+        """replay CG iteration chunks from HIP graphs (results unchanged)"""
+        _check(lib().cpfft_set_graphs(self.h, int(on)), "cpfft_set_graphs")
+
     def set_timing(self, on):
         lib().cpfft_set_timing(self.h, int(on))
 
@@ -308,3 +332,38 @@ class Solver:
                                     maxIter, int(verbose), _p(Pbar), cnt.ctypes.data_as(_ip),
                                     _p(times), int(mlp_log_last), int(mlp_log_stride))
         return rc, Pbar, cnt, times
+
+    def driver_state(self):
+        """(barF_t[9], P_bar[9], C_homo[81]) the driver carries between steps"""
+        p = lib().cpfft_driver_state(self.h)
+        a = np.ctypeslib.as_array(p, shape=(99,)).copy()
+        return a[0:9], a[9:18], a[18:99]
+
+
+def compute_batch(jobs, nthreads=8):
+    """Batched uneven RVEs (cpfft_compute_batch): jobs is a list of
+    (solver, BC_all, isNBC, tolPCG, tolNR, maxIter, first_step).  Runs
+    them on nthreads host threads, heaviest first, each RVE on its own
+    context and stream.  Returns (rc, [(rc, Pbar, counters, times), ...])
+    in the order of jobs."""
+    n = len(jobs)
+    items = (CBatchItem * max(n, 1))()
+    keep = []
+    out = []
+    for k, (s, BC_all, isNBC, tolPCG, tolNR, maxIter, first_step) in enumerate(jobs):
+        BC = np.ascontiguousarray(BC_all, dtype=np.float64)
+        nbc = np.ascontiguousarray(isNBC, dtype=np.int32)
+        nstep = BC.shape[0]
+        Pbar = np.zeros((nstep, 9))
+        cnt = np.zeros((nstep, 4), dtype=np.int32)
+        keep += [BC, nbc]
+        out.append((Pbar, cnt))
+        it = items[k]
+        it.ctx, it.first_step, it.nstep = s.h, first_step, nstep
+        it.BC_all, it.isNBC = _p(BC), nbc.ctypes.data_as(_ip)
+        it.tolPCG, it.tolNR, it.maxIter = tolPCG, tolNR, maxIter
+        it.weight = float(s.N) ** 3 * nstep
+        it.Pbar_out, it.counters = _p(Pbar), cnt.ctypes.data_as(_ip)
+    rc = driver().cpfft_compute_batch(items, n, int(nthreads))
+    res = [(items[k].rc, out[k][0], out[k][1], np.array(items[k].times[:])) for k in range(n)]
+    return rc, res

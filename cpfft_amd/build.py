@@ -1,7 +1,8 @@
 """Build the MI355X library and its Fortran driver, in-tree.
 
   libcpfft_amd.so     HIP kernels + C-ABI (include/cpfft.h), gfx950 only
-  libcpfft_driver.so  Fortran load-step driver (cpfft_compute), amdflang
+  libcpfft_driver.so  Fortran load-step driver (cpfft_compute), amdflang, and
+                      the C++ batch runner (cpfft_compute_batch)
 
 Run `python -m cpfft_amd.build` (or __graft_entry__.build()).
 """
@@ -45,13 +46,17 @@ def build_hip():
 
 
 def build_fortran():
+    """The Fortran load-step driver and the C++ batch runner above it."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     os.makedirs(BUILD, exist_ok=True)
     out = os.path.join(PKG, "libcpfft_driver.so")
     tmp = out + ".tmp"
+    obj = os.path.join(BUILD, "batch_runner.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-c",
+          os.path.join(FSRC, "batch_runner.cpp"), "-o", obj])
     _run([fc, "-O2", "-fPIC", "-shared", "-module-dir", BUILD,
-          os.path.join(FSRC, "cpfft_bindings.f90"), os.path.join(FSRC, "fft_nr3.f90"),
-          "-L" + PKG, "-lcpfft_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+          os.path.join(FSRC, "cpfft_bindings.f90"), os.path.join(FSRC, "fft_nr3.f90"), obj,
+          "-L" + PKG, "-lcpfft_amd", "-lstdc++", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, out)
     return out
 

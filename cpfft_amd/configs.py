@@ -66,6 +66,7 @@ class Workload:
     gamma_mode: int = 1         # 1 textbook projection, 0 reference operator
     material: Material = field(default_factory=Material)
     cycles: int = 0             # > 0: tension-compression cycles of amplitude `strain`
+    direction: tuple = None     # unit n: every F component prescribed, F = I + eps n (x) n
 
     def angles(self):
         return polycrystal(self.N, self.ngrains, self.seed)[1]
@@ -74,11 +75,18 @@ class Workload:
         """(BC_all [nstep][9], isNBC [9]) for uniaxial loading along x:
         monotonic tension to `strain`, or with cycles > 0 a triangle wave
         0 -> +strain -> -strain -> 0 repeated `cycles` times (mults of
-        either sign, as a loading deck may give them)."""
+        either sign, as a loading deck may give them).  With `direction`
+        set, uniaxial strain along n instead: every component of F
+        prescribed (no mixed boundary condition)."""
         isNBC = np.zeros(9, np.int32)
-        isNBC[[4, 8]] = 1
         FP_max = np.zeros(9)
-        FP_max[0] = self.strain
+        if self.direction is not None:
+            # F_bar = I + eps n (x) n, all nine components prescribed
+            n = np.asarray(self.direction, dtype=np.float64)
+            FP_max[:] = self.strain * np.outer(n, n).ravel()
+        else:
+            isNBC[[4, 8]] = 1
+            FP_max[0] = self.strain
         if self.cycles > 0:
             q = self.nstep // (4 * self.cycles)
             assert q * 4 * self.cycles == self.nstep, "nstep must be a multiple of 4 * cycles"
@@ -102,4 +110,40 @@ WORKLOADS = {
     # elastic constants of the FCC material are kept
     "c5_256": Workload("c5_256", 256, 1024, 20261202 + 5, 720, 0.015, cycles=3,
                        material=Material(slip_set=8)),
+}
+
+
+@dataclass
+class BatchWorkload:
+    """Many independent RVEs of uneven size solved together: grid edges
+    drawn from `edges`, grain counts uniform in [grains[0], grains[1]],
+    loading F_bar = I + eps n (x) n along a direction n uniform on the
+    sphere, at a strain rate log-uniform in [rates[0], rates[1]] per second
+    with `dstrain` strain per step (dt = dstrain / rate)."""
+    name: str
+    nrve: int
+    edges: tuple
+    grains: tuple
+    rates: tuple
+    nstep: int
+    dstrain: float
+    seed: int
+
+    def members(self):
+        rng = np.random.default_rng(self.seed)
+        N = rng.choice(np.asarray(self.edges), self.nrve)
+        ng = rng.integers(self.grains[0], self.grains[1] + 1, self.nrve)
+        d = rng.standard_normal((self.nrve, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rate = 10.0 ** rng.uniform(np.log10(self.rates[0]), np.log10(self.rates[1]), self.nrve)
+        return [Workload(f"{self.name}_{k}", int(N[k]), int(ng[k]), self.seed * 100 + k,
+                         self.nstep, self.nstep * self.dstrain, dt=self.dstrain / float(rate[k]),
+                         direction=tuple(d[k])) for k in range(self.nrve)]
+
+
+# 24 RVEs with edges 32-96, 16-128 grains, each with its own loading
+# direction and strain rate in [1e-4, 1e-2]/s, 30 steps of 2.5e-4 strain
+BATCH_WORKLOADS = {
+    "c4_batch": BatchWorkload("c4_batch", 24, (32, 48, 64, 96), (16, 128), (1e-4, 1e-2), 30,
+                              2.5e-4, 20261202 + 4),
 }

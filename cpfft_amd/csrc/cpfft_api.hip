@@ -72,6 +72,12 @@ struct cpfft_ctx {
   int32_t* oid = nullptr;
   long long* dcnt = nullptr;
   long long cg_iters = 0, gkdf_calls = 0, des_calls = 0;
+  double drv[99] = {};  // driver state: barF_t, P_bar, C_homo
+  // CG iteration chunks replayed from HIP graphs (cpfft_set_graphs): one
+  // per solution pointer for cpfft_pcg, one for tangent_homo
+  int use_graphs = 0;
+  std::vector<std::pair<const double*, hipGraphExec_t>> pcg_graphs;
+  hipGraphExec_t homo_graph = nullptr;
   // kernel timing: event pairs recorded on the stream without host
   // synchronisation, resolved by cpfft_kernel_times
   static constexpr int kFam = 4, kRing = 8192;
@@ -121,6 +127,8 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   for (int f = 0; f < cpfft_ctx::kFam; ++f)
     for (int k = 0; k < 2; ++k)
       for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
+  for (auto& pg : c->pcg_graphs) (void)hipGraphExecDestroy(pg.second);
+  if (c->homo_graph) (void)hipGraphExecDestroy(c->homo_graph);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -266,7 +274,15 @@ extern "C" int cpfft_sync(cpfft_ctx* c) {
   return 0;
 }
 
+static void driver_state_init(cpfft_ctx* c) {
+  std::fill(c->drv, c->drv + 99, 0.0);
+  c->drv[0] = c->drv[4] = c->drv[8] = 1.0;
+}
+
+extern "C" double* cpfft_driver_state(cpfft_ctx* c) { return c ? c->drv : nullptr; }
+
 extern "C" int cpfft_reset(cpfft_ctx* c) {
+  driver_state_init(c);
   const long V = c->V;
   const int ns = c->nslip, W = CPFFT_NWORDS(ns);
   std::vector<double> h(9 * V, 0.0);
@@ -409,6 +425,64 @@ extern "C" int cpfft_mean9(cpfft_ctx* c, const double* x, double out[9]) {
   return 0;
 }
 
+// CG iterations enqueued per host round trip
+static constexpr int kChunk = 8;
+
+// kChunk CG iterations of cpfft_pcg on the solution x: q = A p (pass A also
+// forms p = r + beta p, pass D the p.q partial sums on the planned path),
+// alpha, x and r updates, convergence test.  Every kernel reads the scalars
+// in c->dcg and returns at once after convergence, so the chunk has no
+// host-side control flow and can be replayed from a graph.
+static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
+  const long n = 9 * c->V;
+  SpecGeom g = c->geom;
+  g.skip = &c->dcg->done;
+  const bool fused = gkdf_pencil_supported(c->N);
+  const int ndp = gkdf_dot_parts(c->N);
+  const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart};
+  for (int j = 0; j < kChunk; ++j) {
+    tstart(c, 0);
+    if (fused) {
+      launch_gkdf_pencil(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream, &fz);
+    } else {
+      launch_gkdf(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
+    }
+    tstop(c, 0);
+    tstart(c, 2);
+    if (fused) {
+      launch_cg_alpha(c->dpart, ndp, c->dcg, c->stream);
+      launch_cg_update_check(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
+    } else {
+      launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
+    }
+    tstop(c, 2);
+    if (!fused) launch_cg_dir_dev(n, c->dcg, c->r, c->p, c->stream);
+  }
+}
+
+// record what enqueue() launches on the context's stream into an
+// executable graph (kernels only: no memset or copy nodes)
+template <class F>
+static int capture_chunk(cpfft_ctx* c, F&& enqueue, hipGraphExec_t* out) {
+  hipGraph_t gr = nullptr;
+  HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue();
+  const hipError_t le = hipGetLastError();
+  const hipError_t ee = hipStreamEndCapture(c->stream, &gr);
+  HIP_OK(le);
+  HIP_OK(ee);
+  const hipError_t ie = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  HIP_OK(ie);
+  return 0;
+}
+
+extern "C" int cpfft_set_graphs(cpfft_ctx* c, int on) {
+  if (!c) return fail(-1, "cpfft_set_graphs: null context");
+  c->use_graphs = on ? 1 : 0;
+  return 0;
+}
+
 // fftPcg (FFT_nr3.f:214-360): textbook unpreconditioned CG, x0 = 0
 extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, int* iters,
                          double* relres) {
@@ -439,36 +513,26 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   h.done = 0;
   *c->hcg = h;
   HIP_OK(hipMemcpyAsync(c->dcg, c->hcg, sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
-  SpecGeom g = c->geom;
-  g.skip = &c->dcg->done;
-  const bool fused = gkdf_pencil_supported(c->N);
-  // pass A forms p = r + beta p, pass D the p.q partial sums
-  const int ndp = gkdf_dot_parts(c->N);
-  const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart};
-  const int chunk = 8;
+  const bool graph = c->use_graphs && !c->timing;
+  hipGraphExec_t ge = nullptr;
+  if (graph) {
+    for (auto& pg : c->pcg_graphs)
+      if (pg.first == x) ge = pg.second;
+    if (!ge) {
+      if ((rc = capture_chunk(c, [&] { enqueue_pcg_chunk(c, x); }, &ge))) return rc;
+      c->pcg_graphs.emplace_back(x, ge);
+    }
+  }
   int launched = 0;
   for (;;) {
-    if (c->timing && (c->ev_n[0] + chunk > cpfft_ctx::kRing || c->ev_n[2] + chunk > cpfft_ctx::kRing))
-      flush_events(c);
-    for (int j = 0; j < chunk; ++j) {
-      tstart(c, 0);
-      if (fused) {
-        launch_gkdf_pencil(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream, &fz);
-      } else {
-        launch_gkdf(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream);
-      }
-      tstop(c, 0);
-      tstart(c, 2);
-      if (fused) {
-        launch_cg_alpha(c->dpart, ndp, c->dcg, c->stream);
-        launch_cg_update_check(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
-      } else {
-        launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
-      }
-      tstop(c, 2);
-      if (!fused) launch_cg_dir_dev(n, c->dcg, c->r, c->p, c->stream);
+    if (graph) {
+      HIP_OK(hipGraphLaunch(ge, c->stream));
+    } else {
+      if (c->timing && (c->ev_n[0] + kChunk > cpfft_ctx::kRing || c->ev_n[2] + kChunk > cpfft_ctx::kRing))
+        flush_events(c);
+      enqueue_pcg_chunk(c, x);
     }
-    launched += chunk;
+    launched += kChunk;
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(c->hcg, c->dcg, sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -538,6 +602,47 @@ extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
 // K4 forms K4 : p_i for every solve still running, then each solve applies
 // the projection and its own CG update.  Each solve's arithmetic is that of
 // fftPcg on its own, so iterates and iteration counts are unchanged.
+// kChunk iterations of tangent_homo's nine CG solves (scalars in c->dcg9):
+// one K4 pass forms K4 : p_i for every running solve (and on the planned
+// path first p_i = r_i + beta_i p_i), then each solve applies the operator
+// without K4 in place and its own alpha, update and test.  Kernels of
+// finished solves return at once, so the chunk can be replayed from a graph.
+static void enqueue_homo_chunk(cpfft_ctx* c) {
+  const long V = c->V, n = 9 * V;
+  double* const* X = c->th[0];
+  double* const* R = c->th[1];
+  double* const* Pd = c->th[2];
+  double* const* Q = c->th[3];
+  const double* K4 = c->fields[CPFFT_K4];
+  Ptr9 pp;
+  Ptr9r rr;
+  for (int i = 0; i < 9; ++i) {
+    pp.p[i] = Pd[i];
+    pp.q[i] = Q[i];
+    rr.r[i] = R[i];
+  }
+  const bool fused = gkdf_pencil_supported(c->N);
+  const int ndp = gkdf_dot_parts(c->N);
+  for (int j = 0; j < kChunk; ++j) {
+    launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream, fused ? &rr : nullptr);
+    for (int i = 0; i < 9; ++i) {
+      SpecGeom g = c->geom;
+      g.skip = &c->dcg9[i].done;
+      // in place: pass A reads all of Q[i] before pass D writes it
+      if (fused) {
+        const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart};
+        launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
+        launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream);
+        launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
+      } else {
+        launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);
+        launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
+        launch_cg_dir_dev(n, c->dcg9 + i, R[i], Pd[i], c->stream);
+      }
+    }
+  }
+}
+
 extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
   const long V = c->V, n = 9 * V;
   const int maxIter = 1000;
@@ -576,36 +681,15 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
     launch_copy(n, R[i], Pd[i], c->stream);
   }
   HIP_OK(hipMemcpyAsync(c->dcg9, c->hcg9, 9 * sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
-  Ptr9 pp;
-  for (int i = 0; i < 9; ++i) {
-    pp.p[i] = Pd[i];
-    pp.q[i] = Q[i];
-  }
-  const bool fused = gkdf_pencil_supported(c->N);
-  const int ndp = gkdf_dot_parts(c->N);
-  Ptr9r rr;
-  for (int i = 0; i < 9; ++i) rr.r[i] = R[i];
-  const int chunk = 8;
+  const bool graph = c->use_graphs && !c->timing;
+  if (graph && !c->homo_graph &&
+      (rc = capture_chunk(c, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
+    return rc;
   for (;;) {
-    for (int j = 0; j < chunk; ++j) {
-      // K4 : p_i for every running solve; with the fused path the direction
-      // update p_i = r_i + beta_i p_i happens in the same pass
-      launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream, fused ? &rr : nullptr);
-      for (int i = 0; i < 9; ++i) {
-        SpecGeom g = c->geom;
-        g.skip = &c->dcg9[i].done;
-        // in place: pass A reads all of Q[i] before pass D writes it
-        if (fused) {
-          const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart};
-          launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
-          launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream);
-          launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
-        } else {
-          launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);
-          launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
-          launch_cg_dir_dev(n, c->dcg9 + i, R[i], Pd[i], c->stream);
-        }
-      }
+    if (graph) {
+      HIP_OK(hipGraphLaunch(c->homo_graph, c->stream));
+    } else {
+      enqueue_homo_chunk(c);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(c->hcg9, c->dcg9, 9 * sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));

@@ -18,6 +18,13 @@ module cpfft_bindings
        type(c_ptr) :: p
      end function cpfft_field
 
+     ! saved locals of FFT_nr3 (FFT_nr3.f:23-34), kept per context
+     function cpfft_driver_state(ctx) bind(C, name="cpfft_driver_state") result(p)
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr) :: p
+     end function cpfft_driver_state
+
      ! G_K_dF (G_K_dF.f:11-19)
      function cpfft_gkdf(ctx, F, GKF, flgK) bind(C, name="cpfft_gkdf") result(rc)
        import :: c_ptr, c_int

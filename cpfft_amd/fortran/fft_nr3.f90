@@ -8,7 +8,10 @@
 ! loop on the compatible correction dFm with at least one iteration, the
 ! post-convergence stress update, the P_bar check, tangent_homo and the
 ! state roll-over.  Fields never leave the device; the host keeps only the
-! 9-vectors and the 9x9 system of NBC_update.
+! 9-vectors and the 9x9 system of NBC_update.  The procedures are recursive
+! (locals on the stack) and the carried state lives in the context, so one
+! thread per context may run cpfft_compute at the same time
+! (batch_runner.cpp).
 module cpfft_driver
   use iso_c_binding
   use cpfft_bindings
@@ -16,18 +19,12 @@ module cpfft_driver
   private
   public :: cpfft_compute
 
-  ! state carried from one load step to the next (FFT_nr3.f:23-34 keeps
-  ! these as saved locals of the routine)
-  real(c_double), save :: barF_t(9) = [1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0]
-  real(c_double), save :: P_bar(9) = 0.0d0
-  real(c_double), save :: C_homo(81) = 0.0d0
-
 contains
 
   ! NBC_update (FFT_nr3.f:375-433): rows of prescribed P take the
   ! homogenised tangent, rows of prescribed F the identity; solved by
   ! Gaussian elimination with partial pivoting (dgesv).
-  subroutine NBC_update(C_homo, DbarF, P_bar, PBC, isNBC, info)
+  recursive subroutine NBC_update(C_homo, DbarF, P_bar, PBC, isNBC, info)
     real(c_double), intent(in)    :: C_homo(9,9), P_bar(9), PBC(9)
     logical, intent(in)           :: isNBC(9)
     real(c_double), intent(inout) :: DbarF(9)
@@ -79,7 +76,7 @@ contains
     DbarF(1:9) = bbb(1:9)
   end subroutine NBC_update
 
-  real(c_double) function wall()
+  recursive real(c_double) function wall()
     integer(8) :: cnt, rate
     call system_clock(cnt, rate)
     wall = dble(cnt) / dble(rate)
@@ -100,7 +97,7 @@ contains
   ! Steps up to mlp_log_last log warm-start training pairs (every
   ! mlp_log_stride-th voxel) of their converged update before the roll-over;
   ! this has no counterpart in the reference and 0 turns it off.
-  integer(c_int) function cpfft_compute(ctx, first_step, nstep, BC_all, isNBC_c, tolPCG, tolNR, &
+  recursive integer(c_int) function cpfft_compute(ctx, first_step, nstep, BC_all, isNBC_c, tolPCG, tolNR, &
        maxIter, verbose, Pbar_out, counters, times, mlp_log_last, mlp_log_stride) &
        bind(C, name="cpfft_compute")
     type(c_ptr), value     :: ctx
@@ -116,7 +113,16 @@ contains
     integer(c_long_long) :: cnt0(8), cnt1(8)
     integer :: step, istep, ii, iiter_EBC, iiter_NBC, info, nnewton, ncalls, nloops
     logical :: isNBC(9), existNBC
+    ! state carried from one load step to the next (FFT_nr3.f:23-34 keeps
+    ! these as saved locals of the routine); here it lives in the context,
+    ! so the routine holds no state of its own and several contexts can be
+    ! driven at once from different threads
+    real(c_double), pointer :: st(:), barF_t(:), P_bar(:), C_homo(:)
 
+    call c_f_pointer(cpfft_driver_state(ctx), st, [99])
+    barF_t => st(1:9)
+    P_bar => st(10:18)
+    C_homo => st(19:99)
     cpfft_compute = 0
     times = 0.0d0
     counters = 0

@@ -99,6 +99,12 @@ int cpfft_download(cpfft_ctx* ctx, int field, double* host);
 /* reset Fn = Fn1 = I, Pn = Pn1 = 0, state = initial (FFT_init.f:153-160) */
 int cpfft_reset(cpfft_ctx* ctx);
 int cpfft_sync(cpfft_ctx* ctx);
+/* Host state a load-step driver carries from one step to the next, the
+ * saved locals of FFT_nr3 (FFT_nr3.f:23-34): barF_t(9), P_bar(9),
+ * C_homo(81), in that order (99 doubles owned by the context, set to
+ * (I, 0, 0) by cpfft_create and cpfft_reset).  Keeping it in the context
+ * lets one driver run several RVEs, one after another or at once.        */
+double* cpfft_driver_state(cpfft_ctx* ctx);
 
 /* ---- operators -------------------------------------------------------- */
 /* G_K_dF (G_K_dF.f:11-19): GKF = Gamma:(K4:F) if flgK else Gamma:F.  F and
@@ -135,6 +141,14 @@ int cpfft_dcopy(cpfft_ctx* ctx, const double* x, double* y);           /* dcopy 
 int cpfft_fill9(cpfft_ctx* ctx, double* x, const double v[9]);
 /* P_bar(ii) = sum_jj Pn1(jj,ii) / N3 (FFT_nr3.f:127-134) */
 int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
+
+/* ---- execution options (no counterpart in the reference) -------------- */
+/* on != 0: cpfft_pcg and cpfft_tangent_homo replay their chunks of eight CG
+ * iterations from HIP graphs captured on first use (one per solution
+ * pointer for cpfft_pcg), one graph launch per host round trip instead of
+ * about 70 kernel launches.  Results are unchanged.  While kernel timing is
+ * on (cpfft_set_timing) the kernels are launched one by one.              */
+int cpfft_set_graphs(cpfft_ctx* ctx, int on);
 
 /* ---- diagnostics ------------------------------------------------------ */
 /* counters since creation: [0] G_K_dF applications, [1] CG iterations,
