@@ -131,6 +131,7 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
     logged during the W warm-up steps)."""
     V = wl.N ** 3
     s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    s.autotune()  # the shapes chosen for the plain run (cached per N)
     stride = max(1, (W * V) // (1 << 21))  # at most ~2M logged pairs
     s.mlp_init(seed=1, capacity=W * ((V + stride - 1) // stride), batch=4096)
     rc, _, _, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1,
@@ -167,9 +168,10 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
 def batch_bench(name, W, K, nthreads):
     """Batched uneven RVEs: the members' load steps W+1..W+K timed twice
     from the same warm-up state, once one RVE after another on one host
-    thread with kernel-by-kernel launches, once through cpfft_compute_batch
-    on nthreads threads with graph-replayed CG chunks.  Returns the JSON
-    record (value: the batched aggregate rate)."""
+    thread with kernel-by-kernel launches and the default FFT launch shapes,
+    once through cpfft_compute_batch on nthreads threads with autotuned
+    shapes and graph-replayed CG chunks.  Returns the JSON record (value:
+    the batched aggregate rate)."""
     bw = BATCH_WORKLOADS[name]
     members = bw.members()
     angles = [w.angles() for w in members]
@@ -178,11 +180,15 @@ def batch_bench(name, W, K, nthreads):
     print(f"[bench] {name}: {len(members)} RVEs, edges {[w.N for w in members]}, "
           f"{vox} voxels in all, warmup={W} steps={K}", flush=True)
     legs = {}
-    for leg, nt, graphs in (("sequential", 1, False), ("batched", nthreads, True)):
+    shapes = {}
+    for leg, nt, graphs, tuned in (("sequential", 1, False, False),
+                                   ("batched", nthreads, True, True)):
         solvers = []
         for w, ang in zip(members, angles):
             sv = C.Solver(w.N, ang, w.material, dt=w.dt, gamma_mode=w.gamma_mode)
             sv.set_graphs(graphs)
+            if tuned:
+                shapes[w.N] = sv.autotune()  # measured once per edge, then cached
             solvers.append(sv)
 
         def jobs(lo, hi, first):
@@ -209,7 +215,7 @@ def batch_bench(name, W, K, nthreads):
             "voxel_updates_per_s": sum(w.N ** 3 * (b[2] - a[2])
                                        for w, a, b in zip(members, c0, c1)) / wall,
             "cg_iterations": int(sum(b[1] - a[1] for a, b in zip(c0, c1))),
-            "threads": nt, "graphs": graphs,
+            "threads": nt, "graphs": graphs, "autotuned": tuned,
             "Pbar": [r[1] for r in res],
             "newton": [int(r[2][:, 0].sum()) for r in res],
         }
@@ -233,6 +239,7 @@ def batch_bench(name, W, K, nthreads):
                          f"{bw.rates[0]}-{bw.rates[1]}/s, steps {W + 1}-{W + K}"),
             "edges": [w.N for w in members], "grains": [w.ngrains for w in members],
             "dt": [w.dt for w in members], "voxels": vox,
+            "fft_launch_shapes": {str(k): v for k, v in sorted(shapes.items())},
         },
         "batched": bat, "sequential": seq,
         "speedup_vs_sequential": seq["wall_s"] / bat["wall_s"],
@@ -267,7 +274,9 @@ def main():
         raise SystemExit(f"{args.workload} has {BC_all.shape[0]} load steps")
     ang = wl.angles()
     s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
-    print(f"[bench] {wl.name}: N={wl.N} grains={wl.ngrains} warmup={W} steps={K}", flush=True)
+    shapes = s.autotune()
+    print(f"[bench] {wl.name}: N={wl.N} grains={wl.ngrains} warmup={W} steps={K} "
+          f"FFT launch shapes {shapes}", flush=True)
 
     rc, Pw, cw, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1)
     if rc != 0:
@@ -300,6 +309,7 @@ def main():
                          f"Voce n=20, uniaxial tension 1e-3/s, dt={wl.dt}s, steps {W + 1}-{nstep}"),
             "grid": wl.N, "grains": wl.ngrains, "gamma_mode": wl.gamma_mode,
             "jac_exact": wl.material.jac_exact, "tolPCG": wl.tolPCG, "tolNR": wl.tolNR,
+            "fft_launch_shapes": shapes,
         },
         "voxel_updates_per_s": V * n_des / wall,
         "per_step": {

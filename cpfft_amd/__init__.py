@@ -43,7 +43,8 @@ EXPORTS = [
     "cpfft_gkdf", "cpfft_pcg",
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
     "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_counters",
-    "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs",
+    "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_autotune",
+    "cpfft_get_tune", "cpfft_set_tune",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
     "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_stats",
 ]
@@ -86,6 +87,9 @@ def lib():
         L.cpfft_field.argtypes = [_vp, ctypes.c_int]
         L.cpfft_field.restype = _vp
         L.cpfft_set_graphs.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.cpfft_get_tune.argtypes = [_vp, _ip]
+        L.cpfft_set_tune.argtypes = [_vp, _ip]
         L.cpfft_driver_state.argtypes = [_vp]
         L.cpfft_driver_state.restype = _dp
         L.cpfft_upload.argtypes = [_vp, ctypes.c_int, _dp]
@@ -260,6 +264,25 @@ class Solver:
     def set_graphs(self, on):
         """replay CG iteration chunks from HIP graphs (results unchanged)"""
         _check(lib().cpfft_set_graphs(self.h, int(on)), "cpfft_set_graphs")
+
+    def autotune(self, reps=10, force=False):
+        """choose the fastest launch shapes of the planned FFT path (results
+        unchanged); returns the shapes in use (see tune())"""
+        rc = lib().cpfft_autotune(self.h, int(reps), int(force))
+        if rc < 0:
+            raise CpfftError(lib().cpfft_last_error().decode())
+        return self.tune()
+
+    def tune(self):
+        """[pencils per block of pass A with K4, without K4, lines per block
+        of the y passes, y lines per block of pass C]"""
+        out = (ctypes.c_int * 4)()
+        _check(lib().cpfft_get_tune(self.h, out), "cpfft_get_tune")
+        return list(out)
+
+    def set_tune(self, shapes):
+        arr = (ctypes.c_int * 4)(*[int(v) for v in shapes])
+        _check(lib().cpfft_set_tune(self.h, arr), "cpfft_set_tune")
 
     def set_timing(self, on):
         lib().cpfft_set_timing(self.h, int(on))

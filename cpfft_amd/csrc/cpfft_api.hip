@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <tuple>
@@ -99,6 +100,8 @@ static int upload_orients(cpfft_ctx* c, const std::vector<std::array<double, 3>>
   return 0;
 }
 
+static void drop_graphs(cpfft_ctx* c);
+
 extern "C" const char* cpfft_last_error(void) { return g_err.c_str(); }
 
 extern "C" int cpfft_destroy(cpfft_ctx* c) {
@@ -127,8 +130,7 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   for (int f = 0; f < cpfft_ctx::kFam; ++f)
     for (int k = 0; k < 2; ++k)
       for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
-  for (auto& pg : c->pcg_graphs) (void)hipGraphExecDestroy(pg.second);
-  if (c->homo_graph) (void)hipGraphExecDestroy(c->homo_graph);
+  drop_graphs(c);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -474,6 +476,100 @@ static int capture_chunk(cpfft_ctx* c, F&& enqueue, hipGraphExec_t* out) {
   const hipError_t ie = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gr);
   HIP_OK(ie);
+  return 0;
+}
+
+static void drop_graphs(cpfft_ctx* c) {
+  for (auto& pg : c->pcg_graphs) (void)hipGraphExecDestroy(pg.second);
+  c->pcg_graphs.clear();
+  if (c->homo_graph) (void)hipGraphExecDestroy(c->homo_graph);
+  c->homo_graph = nullptr;
+}
+
+// ---------------------------------------------------------- autotuner
+// Shapes chosen per (device, N), shared by every context of the process.
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, PencilTune> g_tune;
+
+static void apply_tune(cpfft_ctx* c, const PencilTune& t) {
+  const bool changed = t.a != c->geom.tune.a || t.a0 != c->geom.tune.a0 || t.b != c->geom.tune.b ||
+                       t.c != c->geom.tune.c;
+  c->geom.tune = t;
+  if (changed) drop_graphs(c);  // captured launches carry the old shapes
+}
+
+extern "C" int cpfft_autotune(cpfft_ctx* c, int reps, int force) {
+  if (!c) return fail(-1, "cpfft_autotune: null context");
+  if (!gkdf_pencil_supported(c->N)) return 1;
+  if (reps < 1) reps = 10;
+  const std::pair<int, int> key(c->dev, c->N);
+  if (!force) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      apply_tune(c, it->second);
+      return 0;
+    }
+  }
+  PencilTune t;
+  int* knob[4] = {&t.a, &t.a0, &t.b, &t.c};
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  const double* F = c->fields[CPFFT_DFM];
+  const double* K4 = c->fields[CPFFT_K4];
+  // passes in pipeline order, so each times its variants on the spectrum
+  // the previous pass wrote
+  for (int pass = 0; pass < 4; ++pass) {
+    float best = 0.f;
+    int bi = 0;
+    for (int v = 0; v < 3; ++v) {
+      if (!pencil_candidate(c->N, pass, v)) continue;
+      launch_pencil_pass(c->geom, pass, v, F, K4, c->spec, c->tw, c->stream);  // warm-up
+      HIP_OK(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < reps; ++r)
+        launch_pencil_pass(c->geom, pass, v, F, K4, c->spec, c->tw, c->stream);
+      HIP_OK(hipEventRecord(e1, c->stream));
+      HIP_OK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      if (v == 0 || ms < best) {
+        best = ms;
+        bi = v;
+      }
+    }
+    *knob[pass] = bi;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_OK(hipGetLastError());
+  apply_tune(c, t);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = t;
+  return 0;
+}
+
+extern "C" int cpfft_get_tune(cpfft_ctx* c, int out[4]) {
+  if (!c) return fail(-1, "cpfft_get_tune: null context");
+  const PencilTune& t = c->geom.tune;
+  const int idx[4] = {t.a, t.a0, t.b, t.c};
+  for (int p = 0; p < 4; ++p) out[p] = pencil_candidate(c->N, p, idx[p]);
+  return 0;
+}
+
+extern "C" int cpfft_set_tune(cpfft_ctx* c, const int in[4]) {
+  if (!c) return fail(-1, "cpfft_set_tune: null context");
+  if (!gkdf_pencil_supported(c->N)) return fail(-1, "cpfft_set_tune: no tunable shapes for this N");
+  PencilTune t;
+  int* knob[4] = {&t.a, &t.a0, &t.b, &t.c};
+  for (int p = 0; p < 4; ++p) {
+    int found = -1;
+    for (int v = 0; v < 3 && found < 0; ++v)
+      if (pencil_candidate(c->N, p, v) == in[p] && in[p] > 0) found = v;
+    if (found < 0) return fail(-1, "cpfft_set_tune: shape is not among the candidates for this N");
+    *knob[p] = found;
+  }
+  apply_tune(c, t);
   return 0;
 }
 

@@ -55,13 +55,36 @@ constexpr int tile_z(int n) {
     if (n % t == 0 && t * n <= 1024) best = t;
   return best;
 }
-// lines per block of the y passes: the largest divisor of N up to 8
-constexpr int tile_y(int n) {
-  int best = 1;
-  for (int t = 1; t <= 8; ++t)
-    if (n % t == 0) best = t;
-  return best;
+// Launch-shape candidates of the autotuner (PencilTune): the i-th largest
+// (i = 0, 1, 2) of the admissible values, 0 when there is none.  Every
+// candidate computes bit-identical results: the shapes only change which
+// block transforms which lines, never the arithmetic of a line.
+// pass A: pencils per block, even divisors t of N with t*N <= 1024
+constexpr int tz_cand(int n, int i) {
+  int k = 0;
+  for (int t = n; t >= 2; --t)
+    if (t % 2 == 0 && n % t == 0 && t * n <= 1024) {
+      if (k == i) return t;
+      ++k;
+    }
+  return 0;
 }
+// passes B, B': lines per block, divisors of N up to 8
+constexpr int ty_cand(int n, int i) {
+  int k = 0;
+  for (int t = 8; t >= 1; --t)
+    if (n % t == 0) {
+      if (k == i) return t;
+      ++k;
+    }
+  return 0;
+}
+// pass C: y lines per block (all nine components of each), 1 or 2 where
+// they fit the 64 KiB of static LDS
+constexpr int xl_cand(int n, int i) {
+  return i < 2 && (9 * (i + 1) * lstride(n) + n) * 16 <= 65536 ? i + 1 : 0;
+}
+constexpr int kCands = 3;
 
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
@@ -196,9 +219,9 @@ __device__ __forceinline__ void load_tw(cplx* tw, const cplx* __restrict__ twg) 
   for (int i = threadIdx.x; i < N; i += T) tw[i] = twg[i];
 }
 
-// threads per block of pass A: two voxels per thread with K4 (its registers
-// hold F and two K4 row blocks), four without
-constexpr int threads_z(int n, bool hask) { return hask && tile_z(n) * n / 2 >= 512 ? 512 : 256; }
+// threads per block of pass A for a tile of p voxels: two voxels per thread
+// with K4 (its registers hold F and two K4 row blocks), four without
+constexpr int threads_z(int p, bool hask) { return hask && p / 2 >= 512 ? 512 : 256; }
 
 // ---- pass A: K4:F and real-to-complex FFT along z ------------------------
 // grid (N/TY, N): blockIdx.x = y tile, blockIdx.y = x.  The nine output
@@ -207,8 +230,8 @@ constexpr int threads_z(int n, bool hask) { return hask && tile_z(n) * n / 2 >= 
 // component c is transformed, so the loads overlap the LDS work.
 // DIR: F is the CG direction p, first replaced by p = R + cs->beta p (each
 // element read and written by the same thread).
-template <int N, bool HASK, bool DIR>
-__global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* F,
+template <int N, bool HASK, bool DIR, int TY>
+__global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double* F,
                                                               const double* __restrict__ K4,
                                                               cplx* __restrict__ T1,
                                                               const cplx* __restrict__ twg,
@@ -217,8 +240,8 @@ __global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* F,
                                                               const double* __restrict__ R,
                                                               const CGScalars* __restrict__ cs) {
   if (skip && *skip) return;
-  constexpr int TA = threads_z(N, HASK);
-  constexpr int TY = tile_z(N), P = TY * N, PT = (P + TA - 1) / TA, LP = TY / 2;
+  constexpr int TA = threads_z(TY * N, HASK);
+  constexpr int P = TY * N, PT = (P + TA - 1) / TA, LP = TY / 2;
   constexpr int LS = lstride(N), NZ = N / 2 + 1;
   constexpr int NK = HASK ? 9 : 1;  // values per point loaded per component
   constexpr long V = (long)N * N * N;
@@ -304,12 +327,12 @@ __global__ __launch_bounds__(threads_z(N, HASK)) void kA_zfwd(const double* F,
 // ---- passes B, B': FFT along y with a transpose ---------------------------
 // FWD: T1[c][kz][x][y] -> T2[c][kz][y][x]; inverse: T2 -> T1.
 // grid (N/TX, NZ, 9)
-template <int N, bool FWD>
+template <int N, bool FWD, int TX>
 __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* __restrict__ dst,
                                            const cplx* __restrict__ twg,
                                            const int* __restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int TX = tile_y(N), LS = lstride(N), NZ = N / 2 + 1;
+  constexpr int LS = lstride(N), NZ = N / 2 + 1;
   __shared__ cplx tw[N];
   __shared__ cplx buf[TX * LS];
   const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
@@ -344,13 +367,11 @@ __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* _
 // ---- pass C: FFT along x, Green projection, inverse FFT along x ------------
 // YB consecutive y lines of all nine components per block.
 // grid (N/YB, NZ): blockIdx.x = y / YB, blockIdx.y = kz
-constexpr int ylines_x(int) { return 1; }
-
-template <int N>
+template <int N, int YB>
 __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __restrict__ twg,
                                            int gamma_mode, const int* __restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int LS = lstride(N), NZ = N / 2 + 1, YB = ylines_x(N);
+  constexpr int LS = lstride(N), NZ = N / 2 + 1;
   __shared__ cplx tw[N];
   __shared__ cplx buf[9 * YB * LS];
   const int y0 = blockIdx.x * YB, kz = blockIdx.y;
@@ -430,29 +451,104 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
   }
 }
 
+// pass A with the I-th pencil-count candidate (nothing if there is none)
+template <int N, int I>
+void launch_a(const double* F, const double* K4, cplx* T1, const cplx* tw, const int* sk,
+              hipStream_t s, const CGFuse* fz) {
+  constexpr int TY = tz_cand(N, I);
+  if constexpr (TY > 0) {
+    const dim3 ga(N / TY, N);
+    if (K4 && fz && fz->dir_p)
+      kA_zfwd<N, true, true, TY><<<ga, threads_z(TY * N, true), 0, s>>>(
+          F, K4, T1, tw, sk, fz->dir_p, fz->dir_r, fz->cs);
+    else if (K4)
+      kA_zfwd<N, true, false, TY><<<ga, threads_z(TY * N, true), 0, s>>>(
+          F, K4, T1, tw, sk, nullptr, nullptr, nullptr);
+    else
+      kA_zfwd<N, false, false, TY><<<ga, threads_z(TY * N, false), 0, s>>>(
+          F, nullptr, T1, tw, sk, nullptr, nullptr, nullptr);
+  }
+}
+template <int N, int I>
+void launch_b(bool fwd, const cplx* src, cplx* dst, const cplx* tw, const int* sk, hipStream_t s) {
+  constexpr int TX = ty_cand(N, I), NZ = N / 2 + 1;
+  if constexpr (TX > 0) {
+    if (fwd)
+      kB_y<N, true, TX><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(src, dst, tw, sk);
+    else
+      kB_y<N, false, TX><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(src, dst, tw, sk);
+  }
+}
+template <int N, int I>
+void launch_c(cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s) {
+  constexpr int YB = xl_cand(N, I), NZ = N / 2 + 1;
+  if constexpr (YB > 0) kC_x<N, YB><<<dim3(N / YB, NZ), kT, 0, s>>>(T2, tw, gamma_mode, sk);
+}
+template <int N>
+void pass_a(int i, const double* F, const double* K4, cplx* T1, const cplx* tw, const int* sk,
+            hipStream_t s, const CGFuse* fz) {
+  switch (i) {
+    case 1: launch_a<N, 1>(F, K4, T1, tw, sk, s, fz); break;
+    case 2: launch_a<N, 2>(F, K4, T1, tw, sk, s, fz); break;
+    default: launch_a<N, 0>(F, K4, T1, tw, sk, s, fz); break;
+  }
+}
+template <int N>
+void pass_b(int i, bool fwd, const cplx* src, cplx* dst, const cplx* tw, const int* sk,
+            hipStream_t s) {
+  switch (i) {
+    case 1: launch_b<N, 1>(fwd, src, dst, tw, sk, s); break;
+    case 2: launch_b<N, 2>(fwd, src, dst, tw, sk, s); break;
+    default: launch_b<N, 0>(fwd, src, dst, tw, sk, s); break;
+  }
+}
+template <int N>
+void pass_c(int i, cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s) {
+  switch (i) {
+    case 1: launch_c<N, 1>(T2, tw, gamma_mode, sk, s); break;
+    default: launch_c<N, 0>(T2, tw, gamma_mode, sk, s); break;
+  }
+}
+
+// one pass by itself (autotuner timing): 0 A with K4, 1 A without, 2 B, 3 C
+template <int N>
+void launch_pass(const SpecGeom& g, int pass, int variant, const double* F, const double* K4,
+                 cplx* T1, cplx* T2, const cplx* tw, hipStream_t s) {
+  switch (pass) {
+    case 0: pass_a<N>(variant, F, K4, T1, tw, nullptr, s, nullptr); break;
+    case 1: pass_a<N>(variant, F, nullptr, T1, tw, nullptr, s, nullptr); break;
+    case 2: pass_b<N>(variant, true, T1, T2, tw, nullptr, s); break;
+    default: pass_c<N>(variant, T2, tw, g.gamma_mode, nullptr, s); break;
+  }
+}
+
 template <int N>
 void launch(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* T1, cplx* T2,
             const cplx* tw, hipStream_t s, const CGFuse* fz) {
-  constexpr int TY = tile_z(N), TX = tile_y(N), NZ = N / 2 + 1;
-  static_assert(N % TY == 0 && TY % 2 == 0 && N % TX == 0, "tile shapes");
+  // pass D keeps the default tile: its p.q partial sums, and so their order,
+  // follow that tile
+  constexpr int TY = tile_z(N);
+  static_assert(N % TY == 0 && TY % 2 == 0 && tz_cand(N, 0) == TY, "tile shapes");
   const int* sk = g.skip;
-  const dim3 ga(N / TY, N);
-  if (K4 && fz && fz->dir_p)
-    kA_zfwd<N, true, true><<<ga, threads_z(N, true), 0, s>>>(F, K4, T1, tw, sk, fz->dir_p, fz->dir_r,
-                                                             fz->cs);
-  else if (K4)
-    kA_zfwd<N, true, false><<<ga, threads_z(N, true), 0, s>>>(F, K4, T1, tw, sk, nullptr, nullptr,
-                                                              nullptr);
-  else
-    kA_zfwd<N, false, false><<<ga, threads_z(N, false), 0, s>>>(F, nullptr, T1, tw, sk, nullptr,
-                                                                nullptr, nullptr);
-  kB_y<N, true><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T1, T2, tw, sk);
-  kC_x<N><<<dim3(N / ylines_x(N), NZ), kT, 0, s>>>(T2, tw, g.gamma_mode, sk);
-  kB_y<N, false><<<dim3(N / TX, NZ, 9), kT, 0, s>>>(T2, T1, tw, sk);
+  const PencilTune& t = g.tune;
+  pass_a<N>(K4 ? t.a : t.a0, F, K4, T1, tw, sk, s, fz);
+  pass_b<N>(t.b, true, T1, T2, tw, sk, s);
+  pass_c<N>(t.c, T2, tw, g.gamma_mode, sk, s);
+  pass_b<N>(t.b, false, T2, T1, tw, sk, s);
   if (fz && fz->dot_p)
     kD_zinv<N, true><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, fz->dot_p, fz->dot_part);
   else
     kD_zinv<N, false><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, nullptr, nullptr);
+}
+
+// candidate value v of pass p (0 A with K4, 1 A without, 2 B, 3 C), 0 if none
+template <int N>
+int cand(int p, int v) {
+  switch (p) {
+    case 0: case 1: return v == 0 ? tz_cand(N, 0) : v == 1 ? tz_cand(N, 1) : v == 2 ? tz_cand(N, 2) : 0;
+    case 2: return v == 0 ? ty_cand(N, 0) : v == 1 ? ty_cand(N, 1) : v == 2 ? ty_cand(N, 2) : 0;
+    default: return v == 0 ? xl_cand(N, 0) : v == 1 ? xl_cand(N, 1) : 0;
+  }
 }
 
 template <int N>
@@ -479,6 +575,37 @@ bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, do
     case 96: fp::launch<96>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
     case 128: fp::launch<128>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
     case 256: fp::launch<256>(g, F, K4, out, T1, T2, tw, s, fuse); return true;
+    default: return false;
+  }
+}
+
+int pencil_candidate(int N, int pass, int variant) {
+  if (pass < 0 || pass > 3 || variant < 0 || variant >= fp::kCands) return 0;
+  switch (N) {
+    case 16: return fp::cand<16>(pass, variant);
+    case 32: return fp::cand<32>(pass, variant);
+    case 48: return fp::cand<48>(pass, variant);
+    case 64: return fp::cand<64>(pass, variant);
+    case 96: return fp::cand<96>(pass, variant);
+    case 128: return fp::cand<128>(pass, variant);
+    case 256: return fp::cand<256>(pass, variant);
+    default: return 0;
+  }
+}
+
+bool launch_pencil_pass(const SpecGeom& g, int pass, int variant, const double* F,
+                        const double* K4, cplx* spec, const cplx* tw, hipStream_t s) {
+  if (pencil_candidate(g.N, pass, variant) == 0) return false;
+  cplx* T1 = spec;
+  cplx* T2 = spec + 9 * g.S1;
+  switch (g.N) {
+    case 16: fp::launch_pass<16>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
+    case 32: fp::launch_pass<32>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
+    case 48: fp::launch_pass<48>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
+    case 64: fp::launch_pass<64>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
+    case 96: fp::launch_pass<96>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
+    case 128: fp::launch_pass<128>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
+    case 256: fp::launch_pass<256>(g, pass, variant, F, K4, T1, T2, tw, s); return true;
     default: return false;
   }
 }

@@ -12,6 +12,14 @@
 
 namespace cpfft {
 
+// Launch shapes of the compile-time-planned FFT path chosen by the
+// autotuner, as candidate indices (0 = the default shape): a, a0 pass A
+// with / without K4, b the y passes, c pass C.  All candidates compute
+// bit-identical results.
+struct PencilTune {
+  int a = 0, a0 = 0, b = 0, c = 0;
+};
+
 struct SpecGeom {
   int N, Nz;        // grid edge, number of z frequencies kept (N/2+1)
   long V;           // N^3
@@ -21,6 +29,7 @@ struct SpecGeom {
   int ty;           // kz columns per block in the y pass
   int tz;           // kz columns per block in the x pass
   const int* skip;  // if non-null and *skip != 0, every block returns at once
+  PencilTune tune;  // launch shapes of the planned path
 };
 
 // Gamma:(K4:F) or Gamma:F (K4 == nullptr).  spec: 2*9*S1 complex scratch.
@@ -47,6 +56,12 @@ bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, do
                         cplx* spec, const cplx* tw, hipStream_t s, const CGFuse* fuse = nullptr);
 // number of partial sums pass D writes for a fused dot product
 int gkdf_dot_parts(int N);
+// autotuner support: the value (pencils or lines per block) of candidate
+// `variant` of pass `pass` (0 A with K4, 1 A without, 2 B, 3 C) for edge N,
+// 0 if there is none; and one pass launched by itself with that candidate
+int pencil_candidate(int N, int pass, int variant);
+bool launch_pencil_pass(const SpecGeom& g, int pass, int variant, const double* F,
+                        const double* K4, cplx* spec, const cplx* tw, hipStream_t s);
 
 // alpha = rr / sum(part[0..np)) (the p.q partial sums of a fused pass D)
 void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s);

@@ -150,6 +150,23 @@ int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
  * on (cpfft_set_timing) the kernels are launched one by one.              */
 int cpfft_set_graphs(cpfft_ctx* ctx, int on);
 
+/* Autotuner of the compile-time-planned FFT path (N in 16, 32, 48, 64, 96,
+ * 128, 256): times up to three launch shapes of each of pass A with K4,
+ * pass A without K4, the y passes and pass C on this context's buffers
+ * (reps launches each) and keeps the fastest.  The choice is cached per
+ * device and N for every context of the process (force != 0 measures
+ * again).  Only shapes that compute bit-identical results are candidates
+ * (which block transforms which lines; never the radix order or a
+ * reduction), so tuning changes speed, not results.  Returns 1 (nothing to
+ * tune) for other N.                                                      */
+int cpfft_autotune(cpfft_ctx* ctx, int reps, int force);
+/* shapes in use: out[0] pencils per block of pass A with K4, out[1]
+ * without K4, out[2] lines per block of the y passes, out[3] y lines per
+ * block of pass C (0 where N has no planned path)                         */
+int cpfft_get_tune(cpfft_ctx* ctx, int out[4]);
+/* set them (each must be one of the candidates; -1 otherwise)             */
+int cpfft_set_tune(cpfft_ctx* ctx, const int in[4]);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* counters since creation: [0] G_K_dF applications, [1] CG iterations,
  * [2] drive_eps_sig calls, [3] local predictor iterations, [4] local

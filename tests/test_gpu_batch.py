@@ -133,3 +133,52 @@ def test_batch_continues_runs():
         assert np.array_equal(np.vstack([r1[k][1], r2[k][1]]), full[k])
     for s in solvers:
         s.close()
+
+
+@pytest.mark.parametrize("N", [16, 32, 48, 64, 96])
+def test_every_tuned_shape_gives_identical_results(N):
+    """The autotuner's candidates (cpfft_set_tune accepts exactly those)
+    change only which block transforms which lines: G_K_dF with and without
+    K4 and a fused CG solve are bitwise equal under every one of them."""
+    s = _solver(N)
+    s.drive_eps_sig(1, 0)
+    rng = np.random.default_rng(3 * N)
+    F = rng.standard_normal((9, N ** 3))
+    default = s.tune()
+    assert all(v > 0 for v in default)
+
+    def run():
+        s.upload(C.DFM, F)
+        s.gkdf(C.DFM, C.B, 0)
+        g0 = s.download(C.B)
+        s.gkdf(C.DFM, C.B, 1)
+        g1 = s.download(C.B)
+        s.dscal(-1e-3, C.B)
+        rc, it, _ = s.pcg(C.B, C.WORK, 1e-8)
+        assert rc == 0
+        return g0, g1, it, s.download(C.WORK)
+
+    ref = run()
+    tried = 0
+    for k in range(4):
+        for v in range(1, 65):
+            shape = list(default)
+            shape[k] = v
+            if v == default[k]:
+                continue
+            try:
+                s.set_tune(shape)
+            except C.CpfftError:
+                continue
+            assert s.tune() == shape
+            print(f"\n[tune] N={N} shape {shape}", flush=True)
+            got = run()
+            tried += 1
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+            assert got[2] == ref[2] and np.array_equal(got[3], ref[3])
+    assert tried >= 3
+    chosen = s.autotune(reps=3, force=True)
+    print(f"\n[tune] N={N} default {default} autotuned {chosen}", flush=True)
+    got = run()
+    assert np.array_equal(got[1], ref[1]) and np.array_equal(got[3], ref[3])
+    s.close()
