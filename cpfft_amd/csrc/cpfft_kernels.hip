@@ -491,7 +491,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars*
   long chunk = (n + gridDim.x - 1) / gridDim.x;
   long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
   double acc = 0.0;
-  for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+  // four elements per pass with their loads issued together (more bytes in
+  // flight per wave); the sum keeps the element order
+  long i = lo + threadIdx.x;
+  for (; i + 3 * kBlock < hi; i += 4 * kBlock) {
+    double pv[4], qv[4], xv[4], rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      pv[u] = p[i + u * kBlock];
+      qv[u] = q[i + u * kBlock];
+      xv[u] = x[i + u * kBlock];
+      rv[u] = r[i + u * kBlock];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[i + u * kBlock] = xv[u] + a * pv[u];
+      const double ri = rv[u] - a * qv[u];
+      r[i + u * kBlock] = ri;
+      acc += ri * ri;
+    }
+  }
+  for (; i < hi; i += kBlock) {
     x[i] = x[i] + a * p[i];
     double ri = r[i] - a * q[i];
     r[i] = ri;

@@ -427,17 +427,32 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
     line[pidx(k)] = cplx{A.re - B.im, A.im + B.re};
     if (k >= 1 && N - k >= NZ) line[pidx(N - k)] = cplx{A.re + B.im, B.re - A.im};
   }
+  const long e0 = ((long)x * N + y0) * N;
+  // DOT: this thread's direction values, loaded before the FFT so that
+  // their latency overlaps it
+  constexpr int PP = (P + kT - 1) / kT;
+  double pv[DOT ? PP : 1];
+  if constexpr (DOT) {
+#pragma unroll
+    for (int j = 0; j < PP; ++j) {
+      const int p = threadIdx.x + j * kT;
+      pv[j] = p < P ? Pd[c * V + e0 + p] : 0.0;
+    }
+  }
   __syncthreads();
   fft<N, LP, false>(buf, tw);
   const double sc = 1.0 / (double)V;
-  const long e0 = ((long)x * N + y0) * N;
   double acc = 0.0;
-  for (int p = threadIdx.x; p < P; p += kT) {
-    const int yy = p / N, z = p - yy * N;
-    const cplx v = buf[(yy >> 1) * LS + pidx(z)];
-    const double q = ((yy & 1) ? v.im : v.re) * sc;
-    out[c * V + e0 + p] = q;
-    if constexpr (DOT) acc += q * Pd[c * V + e0 + p];
+#pragma unroll
+  for (int j = 0; j < PP; ++j) {
+    const int p = threadIdx.x + j * kT;
+    if (p < P) {
+      const int yy = p / N, z = p - yy * N;
+      const cplx v = buf[(yy >> 1) * LS + pidx(z)];
+      const double q = ((yy & 1) ? v.im : v.re) * sc;
+      out[c * V + e0 + p] = q;
+      if constexpr (DOT) acc += q * pv[j];
+    }
   }
   if constexpr (DOT) {
     __shared__ double red[kT];
