@@ -571,20 +571,35 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
   double k[81];
 #pragma unroll
   for (int j = 0; j < 81; ++j) k[j] = K4[j * V + e];
-  for (int i = 0; i < nrhs; ++i) {
-    if (cs[i].done) continue;
+  // r and p of right-hand side i in buffer i % 2: those of the next running
+  // one are loaded before the current one is computed and stored, so their
+  // latency overlaps the work (the done flags are the same in every thread)
+  auto running = [&](int i) { return i < nrhs && cs[i].done == 0; };
+  double rb[2][9], pb[2][9];
+  auto load = [&](int i, double (&r9)[9], double (&p9)[9]) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      if (DIR) r9[c] = rr.r[i][c * V + e];
+      p9[c] = pp.p[i][c * V + e];
+    }
+  };
+  if (running(0)) load(0, rb[0], pb[0]);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    if (i + 1 < 9 && running(i + 1)) load(i + 1, rb[(i + 1) & 1], pb[(i + 1) & 1]);
+    if (!running(i)) continue;
     double f[9];
     if (DIR) {
       const double beta = cs[i].beta;
       double* pw = const_cast<double*>(pp.p[i]);
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
-        f[c] = rr.r[i][c * V + e] + beta * pw[c * V + e];
+        f[c] = rb[i & 1][c] + beta * pb[i & 1][c];
         pw[c * V + e] = f[c];
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < 9; ++c) f[c] = pp.p[i][c * V + e];
+      for (int c = 0; c < 9; ++c) f[c] = pb[i & 1][c];
     }
 #pragma unroll
     for (int r = 0; r < 9; ++r) pp.q[i][r * V + e] = ddot42n_row(k + 9 * r, f);
