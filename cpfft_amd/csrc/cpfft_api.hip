@@ -66,7 +66,9 @@ struct cpfft_ctx {
   MlpBuffers mlp;
   bool mlp_ready = false;  // buffers allocated
   int mlp_on = 0;
-  long mlp_count = 0;      // logged pairs
+  long mlp_count = 0;      // pairs held in the log (at most its capacity)
+  long mlp_total = 0;      // pairs ever logged; the log is a ring over the last ones
+  long mlp_cap = 0;        // ring size (the capacity of the last cpfft_mlp_init)
   long mlp_steps = 0;      // Adam steps taken
   long long ws_updates = 0;
   void* orients = nullptr;
@@ -883,6 +885,8 @@ extern "C" int cpfft_mlp_init(cpfft_ctx* c, uint64_t seed, long capacity, int ba
   for (int i = 0; i < kMlpOut; ++i) nrm[2 * kMlpIn + kMlpOut + i] = 1.0f;
   HIP_OK(hipMemcpy(b.norm, nrm.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
   c->mlp_count = 0;
+  c->mlp_total = 0;
+  c->mlp_cap = capacity;
   c->mlp_steps = 0;
   c->mlp_on = 0;
   return 0;
@@ -892,15 +896,23 @@ extern "C" int cpfft_mlp_log(cpfft_ctx* c, int step, long stride) {
   if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_log: cpfft_mlp_init first");
   if (stride < 1) return fail(-1, "cpfft_mlp_log: stride must be >= 1");
   MlpBuffers& b = c->mlp;
-  long rows = (c->V + stride - 1) / stride;
-  rows = std::min(rows, b.cap - c->mlp_count);
-  if (rows <= 0) return 0;  // log full
-  launch_mlp_features(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
-                      c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
-                      c->fields[CPFFT_STATE_NP1], 0, stride, rows, b.logX + c->mlp_count * kMlpIn,
-                      b.logT + c->mlp_count * kMlpOut, c->stream);
+  // every stride-th voxel (at most one log's worth), written at the ring
+  // position after the last pair logged, wrapping over the oldest pairs
+  const long cap = c->mlp_cap;
+  const long rows = std::min((c->V + stride - 1) / stride, cap);
+  long pos = c->mlp_total % cap, done = 0;
+  while (done < rows) {
+    const long n = std::min(rows - done, cap - pos);
+    launch_mlp_features(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
+                        c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                        c->fields[CPFFT_STATE_NP1], done * stride, stride, n,
+                        b.logX + pos * kMlpIn, b.logT + pos * kMlpOut, c->stream);
+    done += n;
+    pos = (pos + n) % cap;
+  }
   HIP_OK(hipGetLastError());
-  c->mlp_count += rows;
+  c->mlp_total += rows;
+  c->mlp_count = std::min(c->mlp_total, cap);
   return 0;
 }
 

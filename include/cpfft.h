@@ -201,7 +201,9 @@ int cpfft_set_timing(cpfft_ctx* ctx, int on);
  * `capacity` training pairs and work buffers for batches of `batch` rows  */
 int cpfft_mlp_init(cpfft_ctx* ctx, uint64_t seed, long capacity, int batch);
 /* append (inputs, converged slip increments) of every stride-th voxel of
- * the current update (FN, FN1, STATE_N -> STATE_NP1) to the log           */
+ * the current update (FN, FN1, STATE_N -> STATE_NP1) to the log; the log is
+ * a ring of `capacity` pairs, so once full the newest pairs replace the
+ * oldest (online retraining on recent load steps)                         */
 int cpfft_mlp_log(cpfft_ctx* ctx, int step, long stride);
 /* normalise the log, then `iters` Adam steps (learning rate lr) on batches
  * drawn with replacement; *loss = mean squared error (normalised units) of

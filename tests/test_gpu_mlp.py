@@ -100,3 +100,42 @@ def test_warm_start_end_to_end():
     assert local_b < 1.5 * local_a
     a.close()
     b.close()
+
+
+def test_online_retraining_on_a_ring_log():
+    """The surrogate retrained every R load steps on the pairs the method
+    logged in the latest steps (the log is a ring smaller than one round of
+    pairs, so it wraps), warm-starting the steps in between.  The FP64 local
+    Newton stays the judge: P_bar agrees with the plain run to 1e-6 and the
+    global Newton counts are equal."""
+    N, R, rounds = 16, 2, 3
+    nstep = R * rounds
+    wl = C.Workload("online", N, 8, 20261202 + 12, nstep, nstep * 2.5e-4)
+    BC, isNBC = wl.bc()
+    ang = wl.angles()
+    args = (isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    a = C.Solver(N, ang, wl.material, dt=wl.dt, gamma_mode=1)
+    rc, Pa, cnta, _ = a.compute(BC, *args)
+    assert rc == 0
+    b = C.Solver(N, ang, wl.material, dt=wl.dt, gamma_mode=1)
+    cap = 3 * N ** 3 // 2
+    b.mlp_init(seed=1, capacity=cap, batch=512)
+    Ps, cnts = [], []
+    for k in range(rounds):
+        rc, P, cnt, _ = b.compute(BC[k * R:(k + 1) * R], *args, first_step=k * R + 1,
+                                  mlp_log_last=(k + 1) * R)
+        assert rc == 0
+        Ps.append(P)
+        cnts.append(cnt)
+        st = b.mlp_stats()
+        assert st[0] == min((k + 1) * R * N ** 3, cap)
+        loss = b.mlp_train(300, lr=2e-3)
+        print(f"\n[online] round {k}: pairs held {st[0]}, loss {loss:.4f}", flush=True)
+        assert np.isfinite(loss)
+        b.mlp_warm_start(True)
+    Pb, cntb = np.vstack(Ps), np.vstack(cnts)
+    assert abs(Pb - Pa).max() / abs(Pa).max() < 1e-6
+    assert np.array_equal(cntb[:, 0], cnta[:, 0])
+    assert b.mlp_stats()[2] > 0
+    a.close()
+    b.close()
