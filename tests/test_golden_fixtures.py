@@ -1,0 +1,31 @@
+"""CPU checks of the committed golden fixtures (tests/golden/): each holds
+the inputs of the workload it names, satisfies the loading it encodes, and
+is what the oracle computes now (its first steps recomputed bitwise)."""
+import os
+
+import numpy as np
+
+import oracle_lib as O
+from cpfft_amd.configs import WORKLOADS
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_c1_16_fixture():
+    g = np.load(os.path.join(GOLDEN, "c1_16_oracle.npz"))
+    wl = WORKLOADS["c1_16"]
+    BC, isNBC = wl.bc()
+    assert np.array_equal(g["BC"], BC) and np.array_equal(g["isNBC"], isNBC)
+    assert np.array_equal(g["angles"], wl.angles())
+    Pb = g["Pbar"]
+    assert Pb.shape == (wl.nstep, 9) and np.isfinite(Pb).all()
+    # uniaxial tension under mixed BC: P_yy = P_zz = 0 to the mixed-BC
+    # tolerance, P_xx rising
+    assert abs(Pb[:, [4, 8]]).max() <= 2 * wl.tolNR * abs(Pb[:, 0]).max()
+    assert (np.diff(Pb[:, 0]) > 0).all()
+    # the oracle gives the same first two steps today
+    oc = O.crystal(jac_exact=wl.material.jac_exact)
+    P2, c2, rc, _ = O.fft_nr3(oc, wl.N, g["angles"], wl.dt, BC[:2], isNBC, wl.tolPCG, wl.tolNR,
+                              wl.maxIter, gamma_mode=wl.gamma_mode)
+    assert rc == 0
+    assert np.array_equal(P2, Pb[:2]) and np.array_equal(c2, g["counters"][:2])

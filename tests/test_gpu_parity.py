@@ -12,6 +12,8 @@ Tolerances (FP64):
   * load-step loop: volume-averaged P per step 1e-9 relative, identical
     Newton / CG / mixed-BC counts.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -163,6 +165,35 @@ def test_load_steps(N, mode, nstep):
     assert rc == 0 and rco == 0
     assert np.array_equal(cnt[:, [0, 2, 3]], cnto[:, [0, 2, 3]])
     assert rel(Pbar, Pbo) < 1e-9
+    s.close()
+
+
+def test_full_run_c1_16_against_oracle_fixture():
+    """The 16^3, 8-grain workload (configs c1_16) run to its end, 80 load
+    steps of mixed-BC tension to 2%, through the Fortran driver, against the
+    oracle's run of the same inputs stored in tests/golden/c1_16_oracle.npz
+    (made by tests/golden/make_c1_16_oracle.py): the volume-averaged P of
+    every step to 1e-9 relative, the same Newton / mixed-BC / update counts
+    per step, and the final per-voxel first Piola stress, unrotated Cauchy
+    stress, plastic rotation Rp, slip resistance, slip increments and
+    cumulative slips on every 8th voxel to 1e-9 of each field's largest
+    value."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "c1_16_oracle.npz"))
+    wl = C.WORKLOADS["c1_16"]
+    N = int(g["N"])
+    s = C.Solver(N, g["angles"], wl.material, dt=float(g["dt"]), gamma_mode=int(g["gamma_mode"]))
+    print(f"\n[c1_16] {g['BC'].shape[0]} load steps at {N}^3", flush=True)
+    rc, Pbar, cnt, _ = s.compute(g["BC"], g["isNBC"], wl.tolPCG, wl.tolNR, wl.maxIter)
+    assert rc == 0
+    assert np.array_equal(cnt[:, [0, 2, 3]], g["counters"][:, [0, 2, 3]])
+    assert rel(Pbar, g["Pbar"]) < 1e-9
+    idx = g["idx"]
+    assert rel(s.download(C.PN)[:, idx], g["P"]) < 1e-9
+    sg, so = s.download(C.STATE_N)[:, idx], g["state"]
+    ns = s.nslip
+    for w0, w1 in [(0, 6), (6, 15), (15, 16), (23, 23 + ns), (23 + ns, 23 + 2 * ns)]:
+        assert rel(sg[w0:w1], so[w0:w1]) < 1e-9, w0
     s.close()
 
 
