@@ -14,6 +14,9 @@ Operators (the reference routines they replace):
                      the alpha / x, r update / residual-check kernels
   tangent_homo       tangent_homo.f:11-73 (with --homo): the nine CG solves
                      run together, host wall time of one call
+  warm-start network (with --mlp; no counterpart in the reference): forward
+                     pass and loss gradient of the BF16-MFMA network on 65536
+                     random rows, host wall time per call
 
 Prints one JSON line with the average ms and the algorithmic HBM GB/s of each
 (bytes per voxel: 792 with K4, 144 without, 144 for the copy, 936 for the
@@ -25,7 +28,7 @@ import time
 
 import numpy as np
 
-from . import B, DFM, FN1, WORK, Solver
+from . import B, DFM, FN1, MLP_IN, MLP_OUT, WORK, Solver
 from .configs import WORKLOADS
 
 BYTES_GKDF_K4 = 792.0   # read dF 72 + K4 648, write 72
@@ -41,6 +44,7 @@ def main():
     ap.add_argument("--skip-material", action="store_true")
     ap.add_argument("--cg", action="store_true", help="also time one CG solve")
     ap.add_argument("--homo", action="store_true", help="also time one tangent_homo call")
+    ap.add_argument("--mlp", action="store_true", help="also time the warm-start network")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     V = wl.N ** 3
@@ -97,6 +101,20 @@ def main():
         res["homo_wall_s"] = time.perf_counter() - t0
         res["homo_cg_iterations"] = int(s.counters()[1] - c0[1])
         res["homo_ms_per_rhs_iteration"] = res["homo_wall_s"] * 1e3 / max(1, res["homo_cg_iterations"])
+    if args.mlp:
+        rows = 65536
+        s.mlp_init(seed=1, capacity=1, batch=rows)
+        rng = np.random.default_rng(2)
+        X = rng.standard_normal((rows, MLP_IN)).astype(np.float32)
+        T = rng.standard_normal((rows, MLP_OUT)).astype(np.float32)
+        s.mlp_grad(X, T)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.reps):
+            s.mlp_grad(X, T)
+        s.sync()
+        res["mlp_grad_ms_wall"] = (time.perf_counter() - t0) * 1e3 / args.reps
+        res["mlp_rows"] = rows
     if not args.skip_material:
         s.upload(FN1, np.tile(np.array([1.0005, 0, 0, 0, 0.99985, 0, 0, 0, 0.99985])[:, None],
                               (1, V)))
