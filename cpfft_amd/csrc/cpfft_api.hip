@@ -61,6 +61,11 @@ struct cpfft_ctx {
   // tangent_homo's nine simultaneous solves: x, r, p, q per right-hand side
   double* th[4][9] = {};
   CGScalars* dcg9 = nullptr;
+  // on the planned path tangent_homo keeps no x: per right-hand side i,
+  // hdc[9i..9i+9) receives the voxel sums of K4:p_i from pass C and
+  // hm[9i..9i+9) the running sum of alpha * mean(K4:p_i) = mean(K4:x_i)
+  double* hdc = nullptr;
+  double* hm = nullptr;
   CGScalars* hcg9 = nullptr;
   // learned warm start
   MlpBuffers mlp;
@@ -103,6 +108,8 @@ static int upload_orients(cpfft_ctx* c, const std::vector<std::array<double, 3>>
 }
 
 static void drop_graphs(cpfft_ctx* c);
+static void enqueue_homo_chunk(cpfft_ctx* c);
+static int prepare_graphs(cpfft_ctx* c);
 
 extern "C" const char* cpfft_last_error(void) { return g_err.c_str(); }
 
@@ -121,6 +128,8 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
     for (int i = 0; i < 9; ++i)
       if (c->th[a][i]) (void)hipFree(c->th[a][i]);
   if (c->dcg9) (void)hipFree(c->dcg9);
+  if (c->hdc) (void)hipFree(c->hdc);
+  if (c->hm) (void)hipFree(c->hm);
   {
     MlpBuffers& b = c->mlp;
     void* mb[] = {b.params, b.pbf, b.grads, b.m, b.v, b.norm, b.logX, b.logT, b.Xb, b.Xf, b.Tb,
@@ -234,6 +243,8 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   for (int a = 0; a < 4; ++a)
     for (int i = 0; i < 9; ++i) HIP_OK(hipMalloc(&c->th[a][i], 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->dcg9, 9 * sizeof(CGScalars)));
+  HIP_OK(hipMalloc(&c->hdc, 81 * sizeof(double)));
+  HIP_OK(hipMalloc(&c->hm, 81 * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hcg9, 9 * sizeof(CGScalars), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcnt, 8 * sizeof(long long)));
   HIP_OK(hipMemset(c->dcnt, 0, 8 * sizeof(long long)));
@@ -443,7 +454,7 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
   g.skip = &c->dcg->done;
   const bool fused = gkdf_pencil_supported(c->N);
   const int ndp = gkdf_dot_parts(c->N);
-  const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart};
+  const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart, nullptr};
   for (int j = 0; j < kChunk; ++j) {
     tstart(c, 0);
     if (fused) {
@@ -466,8 +477,14 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
 
 // record what enqueue() launches on the context's stream into an
 // executable graph (kernels only: no memset or copy nodes)
+// Capture and instantiation are serialised over the process: contexts
+// driven from several threads (cpfft_compute_batch) then never capture at
+// the same time.
+static std::mutex g_capture_mu;
+
 template <class F>
 static int capture_chunk(cpfft_ctx* c, F&& enqueue, hipGraphExec_t* out) {
+  std::lock_guard<std::mutex> lk(g_capture_mu);
   hipGraph_t gr = nullptr;
   HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   enqueue();
@@ -497,7 +514,10 @@ static void apply_tune(cpfft_ctx* c, const PencilTune& t) {
   const bool changed = t.a != c->geom.tune.a || t.a0 != c->geom.tune.a0 || t.b != c->geom.tune.b ||
                        t.c != c->geom.tune.c;
   c->geom.tune = t;
-  if (changed) drop_graphs(c);  // captured launches carry the old shapes
+  if (changed) {  // captured launches carry the old shapes
+    drop_graphs(c);
+    (void)prepare_graphs(c);
+  }
 }
 
 extern "C" int cpfft_autotune(cpfft_ctx* c, int reps, int force) {
@@ -575,10 +595,30 @@ extern "C" int cpfft_set_tune(cpfft_ctx* c, const int in[4]) {
   return 0;
 }
 
+// the chunk graphs the load-step driver uses (cpfft_pcg on the DFM field,
+// tangent_homo), captured now rather than on first use, so that a batch of
+// contexts run from several threads only launches graphs
+static int prepare_graphs(cpfft_ctx* c) {
+  if (!c->use_graphs) return 0;
+  double* x = c->fields[CPFFT_DFM];
+  bool have = false;
+  for (auto& pg : c->pcg_graphs) have |= pg.first == x;
+  int rc;
+  if (!have) {
+    hipGraphExec_t ge = nullptr;
+    if ((rc = capture_chunk(c, [&] { enqueue_pcg_chunk(c, x); }, &ge))) return rc;
+    c->pcg_graphs.emplace_back(x, ge);
+  }
+  if (!c->homo_graph &&
+      (rc = capture_chunk(c, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
+    return rc;
+  return 0;
+}
+
 extern "C" int cpfft_set_graphs(cpfft_ctx* c, int on) {
   if (!c) return fail(-1, "cpfft_set_graphs: null context");
   c->use_graphs = on ? 1 : 0;
-  return 0;
+  return prepare_graphs(c);
 }
 
 // fftPcg (FFT_nr3.f:214-360): textbook unpreconditioned CG, x0 = 0
@@ -728,10 +768,14 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
       g.skip = &c->dcg9[i].done;
       // in place: pass A reads all of Q[i] before pass D writes it
       if (fused) {
-        const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart};
+        // pass C: voxel sums of K4:p_i; alpha accumulates mean(K4:x_i), so
+        // the update leaves x_i alone
+        const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart, c->hdc + 9 * i};
         launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
-        launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream);
-        launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
+        launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream, c->hdc + 9 * i, c->hm + 9 * i,
+                        1.0 / (double)V);
+        launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream,
+                               false);
       } else {
         launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);
         launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);
@@ -746,6 +790,7 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
   const int maxIter = 1000;
   const double eps = 2.220446049250313e-16;
   if (tol <= eps || tol >= 1.0) return fail(-1, "cpfft_tangent_homo: improper tolerance");
+  const bool fused = gkdf_pencil_supported(c->N);
   double* const* X = c->th[0];
   double* const* R = c->th[1];
   double* const* Pd = c->th[2];
@@ -775,10 +820,11 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
     h.maxit = maxIter;
     h.done = (n2b[i] <= eps) ? 1 : 0;  // zero right-hand side: dF_i = 0
     c->hcg9[i] = h;
-    launch_scal(n, 0.0, X[i], c->stream);
+    if (!fused) launch_scal(n, 0.0, X[i], c->stream);
     launch_copy(n, R[i], Pd[i], c->stream);
   }
   HIP_OK(hipMemcpyAsync(c->dcg9, c->hcg9, 9 * sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
+  if (fused) HIP_OK(hipMemsetAsync(c->hm, 0, 81 * sizeof(double), c->stream));
   const bool graph = c->use_graphs && !c->timing;
   if (graph && !c->homo_graph &&
       (rc = capture_chunk(c, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
@@ -800,6 +846,18 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
     c->cg_iters += c->hcg9[i].k;
     c->gkdf_calls += c->hcg9[i].k;
     if (c->hcg9[i].done == 2) return fail(-2, "cpfft_tangent_homo: CG did not converge within 1000 iterations");
+  }
+  if (fused) {
+    // C_homo(9(j-1)+i) = mean(K4 : E_i)_j + mean(K4 : x_i)_j, the first
+    // term the mean of K4 component 9j+i, the second accumulated above
+    double hm[81];
+    HIP_OK(hipMemcpyAsync(hm, c->hm, sizeof hm, hipMemcpyDeviceToHost, c->stream));
+    for (int j = 0; j < 9; ++j) {
+      launch_mean9(K4 + (long)9 * j * V, V, c->partials, c->dres, c->stream);
+      if ((rc = read_res(c, 9))) return rc;
+      for (int i = 0; i < 9; ++i) C_homo[9 * j + i] = c->hres[i] + hm[9 * i + j];
+    }
+    return 0;
   }
   for (int i = 0; i < 9; ++i) {
     launch_add_identity_comp(V, i, X[i], c->stream);

@@ -472,13 +472,21 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_s(const double* __restrict_
 }
 // alpha = rr / (p.q) from the partial sums (same order as k_sum_parts)
 __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ part, int np,
-                                                     CGScalars* __restrict__ cs) {
+                                                     CGScalars* __restrict__ cs,
+                                                     const double* __restrict__ dc,
+                                                     double* __restrict__ m, double scale) {
   if (cs->done) return;
   __shared__ double sh[kBlock];
   const double pq = ordered_sum<kBlock>(part, np, sh);
-  if (threadIdx.x == 0) cs->alpha = cs->rr / pq;
+  if (threadIdx.x == 0) {
+    const double a = cs->rr / pq;
+    cs->alpha = a;
+    if (m)
+      for (int c = 0; c < 9; ++c) m[c] += a * (dc[c] * scale);
+  }
 }
 
+template <bool XUPD>
 __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars* __restrict__ cs,
                                                         const double* __restrict__ p,
                                                         const double* __restrict__ q,
@@ -498,21 +506,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars*
     double pv[4], qv[4], xv[4], rv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      pv[u] = p[i + u * kBlock];
+      if constexpr (XUPD) {
+        pv[u] = p[i + u * kBlock];
+        xv[u] = x[i + u * kBlock];
+      }
       qv[u] = q[i + u * kBlock];
-      xv[u] = x[i + u * kBlock];
       rv[u] = r[i + u * kBlock];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      x[i + u * kBlock] = xv[u] + a * pv[u];
+      if constexpr (XUPD) x[i + u * kBlock] = xv[u] + a * pv[u];
       const double ri = rv[u] - a * qv[u];
       r[i + u * kBlock] = ri;
       acc += ri * ri;
     }
   }
   for (; i < hi; i += kBlock) {
-    x[i] = x[i] + a * p[i];
+    if constexpr (XUPD) x[i] = x[i] + a * p[i];
     double ri = r[i] - a * q[i];
     r[i] = ri;
     acc += ri * ri;
@@ -537,19 +547,23 @@ __global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const doubl
 }
 
 void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
-                            double* r, double* partials, hipStream_t s) {
-  k_cg_update_s<<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
+                            double* r, double* partials, hipStream_t s, bool xupd) {
+  if (xupd)
+    k_cg_update_s<true><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
+  else
+    k_cg_update_s<false><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
   k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
 }
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s) {
-  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs);
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s, const double* dc,
+                     double* m, double scale) {
+  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs, dc, m, scale);
 }
 
 void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,
                     double* r, double* partials, hipStream_t s) {
   k_dot_part_s<<<kRedBlocks, kBlock, 0, s>>>(p, q, n, partials, cs);
-  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
-  k_cg_update_s<<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
+  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, nullptr, nullptr, 0.0);
+  k_cg_update_s<true><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
   k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
 }
 void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s) {

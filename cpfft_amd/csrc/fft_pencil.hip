@@ -367,9 +367,13 @@ __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* _
 // ---- pass C: FFT along x, Green projection, inverse FFT along x ------------
 // YB consecutive y lines of all nine components per block.
 // grid (N/YB, NZ): blockIdx.x = y / YB, blockIdx.y = kz
+// dc != nullptr: the block of frequency (0, 0, 0) also stores the real parts
+// of that frequency before the projection, i.e. the voxel sums of the nine
+// input components (the transforms are unscaled).
 template <int N, int YB>
 __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __restrict__ twg,
-                                           int gamma_mode, const int* __restrict__ skip) {
+                                           int gamma_mode, const int* __restrict__ skip,
+                                           double* __restrict__ dc) {
   if (skip && *skip) return;
   constexpr int LS = lstride(N), NZ = N / 2 + 1;
   __shared__ cplx tw[N];
@@ -384,6 +388,11 @@ __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __
   }
   __syncthreads();
   fft<N, 9 * YB, true>(buf, tw);
+  // thread 0 reads (t, x) = (0, 0) before it projects that same point
+  if (dc && blockIdx.x == 0 && kz == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) dc[c] = buf[c * YB * LS + pidx(0)].re;
+  }
   for (int i = threadIdx.x; i < YB * N; i += kT) {
     const int t = i / N, x = i - t * N;
     cplx A[9], out[9];
@@ -495,9 +504,9 @@ void launch_b(bool fwd, const cplx* src, cplx* dst, const cplx* tw, const int* s
   }
 }
 template <int N, int I>
-void launch_c(cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s) {
+void launch_c(cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s, double* dc) {
   constexpr int YB = xl_cand(N, I), NZ = N / 2 + 1;
-  if constexpr (YB > 0) kC_x<N, YB><<<dim3(N / YB, NZ), kT, 0, s>>>(T2, tw, gamma_mode, sk);
+  if constexpr (YB > 0) kC_x<N, YB><<<dim3(N / YB, NZ), kT, 0, s>>>(T2, tw, gamma_mode, sk, dc);
 }
 template <int N>
 void pass_a(int i, const double* F, const double* K4, cplx* T1, const cplx* tw, const int* sk,
@@ -518,10 +527,11 @@ void pass_b(int i, bool fwd, const cplx* src, cplx* dst, const cplx* tw, const i
   }
 }
 template <int N>
-void pass_c(int i, cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s) {
+void pass_c(int i, cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s,
+            double* dc = nullptr) {
   switch (i) {
-    case 1: launch_c<N, 1>(T2, tw, gamma_mode, sk, s); break;
-    default: launch_c<N, 0>(T2, tw, gamma_mode, sk, s); break;
+    case 1: launch_c<N, 1>(T2, tw, gamma_mode, sk, s, dc); break;
+    default: launch_c<N, 0>(T2, tw, gamma_mode, sk, s, dc); break;
   }
 }
 
@@ -548,7 +558,7 @@ void launch(const SpecGeom& g, const double* F, const double* K4, double* out, c
   const PencilTune& t = g.tune;
   pass_a<N>(K4 ? t.a : t.a0, F, K4, T1, tw, sk, s, fz);
   pass_b<N>(t.b, true, T1, T2, tw, sk, s);
-  pass_c<N>(t.c, T2, tw, g.gamma_mode, sk, s);
+  pass_c<N>(t.c, T2, tw, g.gamma_mode, sk, s, fz ? fz->dc : nullptr);
   pass_b<N>(t.b, false, T2, T1, tw, sk, s);
   if (fz && fz->dot_p)
     kD_zinv<N, true><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, fz->dot_p, fz->dot_part);

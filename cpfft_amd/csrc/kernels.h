@@ -39,13 +39,16 @@ struct SpecGeom {
 // dir_p != nullptr: pass A first sets dir_p = dir_r + cs->beta * dir_p (the
 //   CG direction, F must be dir_p) and transforms the new direction;
 // dot_p != nullptr: pass D writes per-block partial sums of dot_p . out
-//   into dot_part (gkdf_dot_parts(N) values, fixed order).
+//   into dot_part (gkdf_dot_parts(N) values, fixed order);
+// dc != nullptr: pass C stores the voxel sums of the nine components of its
+//   input (F, or K4:F) in dc[0..9).
 struct CGFuse {
   double* dir_p;
   const double* dir_r;
   CGScalars* cs;
   const double* dot_p;
   double* dot_part;
+  double* dc;
 };
 
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
@@ -63,12 +66,16 @@ int pencil_candidate(int N, int pass, int variant);
 bool launch_pencil_pass(const SpecGeom& g, int pass, int variant, const double* F,
                         const double* K4, cplx* spec, const cplx* tw, hipStream_t s);
 
-// alpha = rr / sum(part[0..np)) (the p.q partial sums of a fused pass D)
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s);
-// x += alpha p; r -= alpha q with the r.r partial sums, then the convergence
-// test and beta (no p.q, no direction update)
+// alpha = rr / sum(part[0..np)) (the p.q partial sums of a fused pass D);
+// with m != nullptr also m[c] += alpha * dc[c] * scale for c < 9 (the
+// running sum over iterations of alpha times the mean of K4:p)
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s,
+                     const double* dc = nullptr, double* m = nullptr, double scale = 0.0);
+// x += alpha p (xupd; without it x and p are not touched); r -= alpha q
+// with the r.r partial sums, then the convergence test and beta (no p.q, no
+// direction update)
 void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
-                            double* r, double* partials, hipStream_t s);
+                            double* r, double* partials, hipStream_t s, bool xupd = true);
 
 // One CG iteration after q = A p: alpha = rr / (p.q); x += alpha p;
 // r -= alpha q; convergence test on |r|; p = r + beta p.  All scalars stay
