@@ -37,22 +37,25 @@ from cpfft_amd.configs import BATCH_WORKLOADS, WORKLOADS  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # The dominant kernel group is the CG operator of one iteration on the
-# compile-time-planned path: pass A forms the direction p = r + beta p and
-# K4:p, the five passes apply the projection, pass D writes q and the p.q
-# partial sums.  Algorithmic bytes per voxel: read p, r (72 each) and K4
-# (648), write p and q (72 each).
+# compile-time-planned path: pass A forms the direction p = r + beta p, K4:p
+# and, for the textbook (symmetric projection) operator, the p.q partial sums
+# as p.(K4:p); the five passes apply the projection and pass D writes q.
+# Algorithmic bytes per voxel: read p, r (72 each) and K4 (648), write p and
+# q (72 each).
 CG_OPERATOR_BYTES_PER_VOXEL = 72.0 + 72.0 + 648.0 + 72.0 + 72.0
 
 
-def cg_operator_traffic_per_voxel(N):
+def cg_operator_traffic_per_voxel(N, gamma_mode):
     """HBM bytes per voxel the five passes move for that group: pass A reads
     r, p and K4 and writes p and the half spectrum; eight passes over the
-    nine-component half spectrum; pass D reads p again and writes q.  For
-    the operator alone (no CG fusion) the same count gives 1377 B/voxel at
-    128^3, which FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel)
-    reproduce to 0.1%."""
+    nine-component half spectrum; pass D writes q (and, for the reference's
+    even-N operator, reads p again for p.q).  Without the CG work the same
+    count gives 1377 B/voxel for G_K_dF with K4 at 128^3, which
+    FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel) reproduce
+    to 0.1%."""
     nz = N // 2 + 1
-    return 72.0 * 3 + 648.0 + 72.0 * 2 + 8.0 * 144.0 * nz / N
+    reread = 0.0 if gamma_mode == 1 else 72.0
+    return 72.0 * 2 + reread + 648.0 + 72.0 * 2 + 8.0 * 144.0 * nz / N
 
 
 def device_sync():
@@ -322,14 +325,14 @@ def main():
         "kernel_ms": {"cg_operator": avg[0], "drive_eps_sig": avg[1], "cg_update": avg[2],
                       "G_K_dF_no_K4": avg[3]},
         "roofline": {
-            "kernel": ("CG operator: p = r + beta p, G_K_dF with K4, p.q "
+            "kernel": ("CG operator: p = r + beta p, G_K_dF with K4, p.q in pass A "
                        "(5-kernel launch group)"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": cg_operator_traffic_per_voxel(wl.N) * V,
+            "traffic": cg_operator_traffic_per_voxel(wl.N, wl.gamma_mode) * V,
             "algorithmic_bytes_per_launch": CG_OPERATOR_BYTES_PER_VOXEL * V,
             "avg_launch_ms": gk_ms,
             "launches_measured": int(nlaunch[0]),

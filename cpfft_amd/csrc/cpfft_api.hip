@@ -66,6 +66,7 @@ struct cpfft_ctx {
   // hm[9i..9i+9) the running sum of alpha * mean(K4:p_i) = mean(K4:x_i)
   double* hdc = nullptr;
   double* hm = nullptr;
+  double* hpq = nullptr;  // p_i . (K4:p_i) block sums of the K4 pass [9][blocks]
   CGScalars* hcg9 = nullptr;
   // learned warm start
   MlpBuffers mlp;
@@ -130,6 +131,7 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (c->dcg9) (void)hipFree(c->dcg9);
   if (c->hdc) (void)hipFree(c->hdc);
   if (c->hm) (void)hipFree(c->hm);
+  if (c->hpq) (void)hipFree(c->hpq);
   {
     MlpBuffers& b = c->mlp;
     void* mb[] = {b.params, b.pbf, b.grads, b.m, b.v, b.norm, b.logX, b.logT, b.Xb, b.Xf, b.Tb,
@@ -235,7 +237,9 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipMalloc(&c->tmp9, 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->tan6, 36 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->partials, reduce_partials_count(9 * V) * sizeof(double)));
-  HIP_OK(hipMalloc(&c->dpart, std::max(1, gkdf_dot_parts(N)) * sizeof(double)));
+  // p.q partial sums: pass D's (gkdf_dot_parts) or pass A's (N*N/2, one per
+  // pencil pair)
+  HIP_OK(hipMalloc(&c->dpart, std::max({1, gkdf_dot_parts(N), N * N}) * sizeof(double)));
   HIP_OK(hipMalloc(&c->dres, 16 * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcg, sizeof(CGScalars)));
@@ -245,6 +249,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipMalloc(&c->dcg9, 9 * sizeof(CGScalars)));
   HIP_OK(hipMalloc(&c->hdc, 81 * sizeof(double)));
   HIP_OK(hipMalloc(&c->hm, 81 * sizeof(double)));
+  HIP_OK(hipMalloc(&c->hpq, 9L * k4_contract_multi_blocks(V) * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hcg9, 9 * sizeof(CGScalars), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcnt, 8 * sizeof(long long)));
   HIP_OK(hipMemset(c->dcnt, 0, 8 * sizeof(long long)));
@@ -454,7 +459,13 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
   g.skip = &c->dcg->done;
   const bool fused = gkdf_pencil_supported(c->N);
   const int ndp = gkdf_dot_parts(c->N);
-  const CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart, nullptr};
+  // the textbook operator is a symmetric projection: pass A forms p.(K4:p),
+  // which equals p.q; the reference's even-N operator is not, and pass D
+  // forms p.q there
+  const bool apq = fused && c->geom.gamma_mode == 1;
+  const int npq = apq ? c->N * c->N / 2 : ndp;
+  const CGFuse fz{c->p, c->r, c->dcg, apq ? nullptr : c->p, apq ? nullptr : c->dpart, nullptr,
+                  apq ? c->dpart : nullptr};
   for (int j = 0; j < kChunk; ++j) {
     tstart(c, 0);
     if (fused) {
@@ -465,7 +476,7 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
     tstop(c, 0);
     tstart(c, 2);
     if (fused) {
-      launch_cg_alpha(c->dpart, ndp, c->dcg, c->stream);
+      launch_cg_alpha(c->dpart, npq, c->dcg, c->stream);
       launch_cg_update_check(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
     } else {
       launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
@@ -761,8 +772,13 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
   }
   const bool fused = gkdf_pencil_supported(c->N);
   const int ndp = gkdf_dot_parts(c->N);
+  // textbook operator: p_i . q_i = p_i . (K4:p_i) from the K4 pass (see
+  // enqueue_pcg_chunk); the reference's even-N operator: from pass D
+  const bool apq = fused && c->geom.gamma_mode == 1;
+  const int nb = k4_contract_multi_blocks(V);
   for (int j = 0; j < kChunk; ++j) {
-    launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream, fused ? &rr : nullptr);
+    launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream, fused ? &rr : nullptr,
+                             apq ? c->hpq : nullptr);
     for (int i = 0; i < 9; ++i) {
       SpecGeom g = c->geom;
       g.skip = &c->dcg9[i].done;
@@ -770,10 +786,15 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
       if (fused) {
         // pass C: voxel sums of K4:p_i; alpha accumulates mean(K4:x_i), so
         // the update leaves x_i alone
-        const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart, c->hdc + 9 * i};
+        const CGFuse fz{nullptr, nullptr, c->dcg9 + i, apq ? nullptr : Pd[i], apq ? nullptr : c->dpart,
+                        c->hdc + 9 * i, nullptr};
         launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
-        launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream, c->hdc + 9 * i, c->hm + 9 * i,
-                        1.0 / (double)V);
+        if (apq)
+          launch_cg_alpha(c->hpq + (long)i * nb, nb, c->dcg9 + i, c->stream, c->hdc + 9 * i,
+                          c->hm + 9 * i, 1.0 / (double)V);
+        else
+          launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream, c->hdc + 9 * i, c->hm + 9 * i,
+                          1.0 / (double)V);
         launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream,
                                false);
       } else {

@@ -576,15 +576,16 @@ void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, 
 template <bool DIR>
 __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const double* __restrict__ K4,
                                                               Ptr9 pp, const CGScalars* __restrict__ cs,
-                                                              int nrhs, Ptr9r rr) {
+                                                              int nrhs, Ptr9r rr,
+                                                              double* __restrict__ pqpart) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= V) return;
+  const bool valid = e < V;  // no early return: pqpart sums use a block barrier
   bool any = false;
   for (int i = 0; i < nrhs; ++i) any |= cs[i].done == 0;
-  if (!any) return;
+  if (!any) return;  // the same in every thread of the launch
   double k[81];
 #pragma unroll
-  for (int j = 0; j < 81; ++j) k[j] = K4[j * V + e];
+  for (int j = 0; j < 81; ++j) k[j] = valid ? K4[j * V + e] : 0.0;
   // r and p of right-hand side i in buffer i % 2: those of the next running
   // one are loaded before the current one is computed and stored, so their
   // latency overlaps the work (the done flags are the same in every thread)
@@ -593,10 +594,13 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
   auto load = [&](int i, double (&r9)[9], double (&p9)[9]) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      if (DIR) r9[c] = rr.r[i][c * V + e];
-      p9[c] = pp.p[i][c * V + e];
+      if (DIR) r9[c] = valid ? rr.r[i][c * V + e] : 0.0;
+      p9[c] = valid ? pp.p[i][c * V + e] : 0.0;
     }
   };
+  // pqpart: per right-hand side, the wave sums of p_i . (K4:p_i)
+  __shared__ double wred[9][kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (running(0)) load(0, rb[0], pb[0]);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -609,24 +613,43 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
         f[c] = rb[i & 1][c] + beta * pb[i & 1][c];
-        pw[c * V + e] = f[c];
+        if (valid) pw[c * V + e] = f[c];
       }
     } else {
 #pragma unroll
       for (int c = 0; c < 9; ++c) f[c] = pb[i & 1][c];
     }
+    double pq = 0.0;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) pp.q[i][r * V + e] = ddot42n_row(k + 9 * r, f);
+    for (int r = 0; r < 9; ++r) {
+      const double q = ddot42n_row(k + 9 * r, f);
+      if (valid) pp.q[i][r * V + e] = q;
+      pq += q * f[r];
+    }
+    if (pqpart) {
+      // butterfly over the wave: every lane ends with the same sum
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) pq += __shfl_xor(pq, off);
+      if (lane == 0) wred[i][wave] = pq;
+    }
+  }
+  if (pqpart) {
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (running(i))
+      pqpart[(long)i * gridDim.x + blockIdx.x] = (wred[i][0] + wred[i][1]) + (wred[i][2] + wred[i][3]);
   }
 }
 
+int k4_contract_multi_blocks(long V) { return (int)((V + kBlock - 1) / kBlock); }
+
 void launch_k4_contract_multi(long V, const double* K4, const Ptr9& pp, const CGScalars* cs,
-                              int nrhs, hipStream_t s, const Ptr9r* rr) {
-  const unsigned g = (unsigned)((V + kBlock - 1) / kBlock);
+                              int nrhs, hipStream_t s, const Ptr9r* rr, double* pqpart) {
+  const unsigned g = (unsigned)k4_contract_multi_blocks(V);
   if (rr)
-    k_k4_contract_multi<true><<<g, kBlock, 0, s>>>(V, K4, pp, cs, nrhs, *rr);
+    k_k4_contract_multi<true><<<g, kBlock, 0, s>>>(V, K4, pp, cs, nrhs, *rr, pqpart);
   else
-    k_k4_contract_multi<false><<<g, kBlock, 0, s>>>(V, K4, pp, cs, nrhs, Ptr9r{});
+    k_k4_contract_multi<false><<<g, kBlock, 0, s>>>(V, K4, pp, cs, nrhs, Ptr9r{}, pqpart);
 }
 
 // ------------------------------------------------------- stress update

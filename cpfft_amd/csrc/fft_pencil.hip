@@ -238,7 +238,8 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
                                                               const int* __restrict__ skip,
                                                               double* Pd,
                                                               const double* __restrict__ R,
-                                                              const CGScalars* __restrict__ cs) {
+                                                              const CGScalars* __restrict__ cs,
+                                                              double* __restrict__ pqp) {
   if (skip && *skip) return;
   constexpr int TA = threads_z(TY * N, HASK);
   constexpr int P = TY * N, PT = (P + TA - 1) / TA, LP = TY / 2;
@@ -286,6 +287,9 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
   double kc[PT][NK], kn[PT][NK];
   load_c(0, kc);
   double* bd = reinterpret_cast<double*>(buf);
+  double pq[PT];  // pqp: F . (K4:F) at this thread's points
+#pragma unroll
+  for (int u = 0; u < PT; ++u) pq[u] = 0.0;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
 #pragma unroll
@@ -295,6 +299,7 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
         double v;
         if constexpr (HASK) {
           v = ddot42n_row(kc[u], f[u]);
+          if (pqp) pq[u] += v * f[u][c];
         } else {
           v = kc[u][0];
         }
@@ -320,6 +325,30 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
       for (int u = 0; u < PT; ++u)
 #pragma unroll
         for (int d = 0; d < NK; ++d) kc[u][d] = kn[u][d];
+    }
+  }
+  if constexpr (HASK) {
+    // one partial sum per pencil pair (y lines y, y+1 of this x: 2N voxels,
+    // local points [2Nm, 2N(m+1))), reduced by a fixed halving order of its
+    // own, so the sums do not depend on the tile of the block
+    if (pqp) {
+      __shared__ double red[P];
+#pragma unroll
+      for (int u = 0; u < PT; ++u) {
+        const int p = threadIdx.x + u * TA;
+        if (inb(p)) red[p] = pq[u];
+      }
+      __syncthreads();
+      for (int len = 2 * N; len > 1;) {
+        const int half = (len + 1) / 2, n2 = len - half;
+        for (int i = threadIdx.x; i < LP * n2; i += TA) {
+          const int m = i / n2, j = i - m * n2;
+          red[2 * N * m + j] += red[2 * N * m + j + half];
+        }
+        __syncthreads();
+        len = half;
+      }
+      for (int m = threadIdx.x; m < LP; m += TA) pqp[(long)x * (N / 2) + y0 / 2 + m] = red[2 * N * m];
     }
   }
 }
@@ -484,13 +513,13 @@ void launch_a(const double* F, const double* K4, cplx* T1, const cplx* tw, const
     const dim3 ga(N / TY, N);
     if (K4 && fz && fz->dir_p)
       kA_zfwd<N, true, true, TY><<<ga, threads_z(TY * N, true), 0, s>>>(
-          F, K4, T1, tw, sk, fz->dir_p, fz->dir_r, fz->cs);
+          F, K4, T1, tw, sk, fz->dir_p, fz->dir_r, fz->cs, fz->pq_part);
     else if (K4)
       kA_zfwd<N, true, false, TY><<<ga, threads_z(TY * N, true), 0, s>>>(
-          F, K4, T1, tw, sk, nullptr, nullptr, nullptr);
+          F, K4, T1, tw, sk, nullptr, nullptr, nullptr, nullptr);
     else
       kA_zfwd<N, false, false, TY><<<ga, threads_z(TY * N, false), 0, s>>>(
-          F, nullptr, T1, tw, sk, nullptr, nullptr, nullptr);
+          F, nullptr, T1, tw, sk, nullptr, nullptr, nullptr, nullptr);
   }
 }
 template <int N, int I>

@@ -41,7 +41,13 @@ struct SpecGeom {
 // dot_p != nullptr: pass D writes per-block partial sums of dot_p . out
 //   into dot_part (gkdf_dot_parts(N) values, fixed order);
 // dc != nullptr: pass C stores the voxel sums of the nine components of its
-//   input (F, or K4:F) in dc[0..9).
+//   input (F, or K4:F) in dc[0..9);
+// pq_part != nullptr (with dir_p): pass A writes the partial sums of
+//   p . (K4:p) over each pencil pair (y, y+1) of each x into
+//   pq_part[x*N/2 + y/2] (N*N/2 values, the same under every tile).  For a
+//   symmetric projection G (the textbook operator) and p in its range, as
+//   every CG direction of a right-hand side G(.) is, p . G(K4:p) equals
+//   p . (K4:p), so pass D then needs no dot product.
 struct CGFuse {
   double* dir_p;
   const double* dir_r;
@@ -49,6 +55,7 @@ struct CGFuse {
   const double* dot_p;
   double* dot_part;
   double* dc;
+  double* pq_part;
 };
 
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
@@ -95,8 +102,12 @@ struct Ptr9 {
 struct Ptr9r {
   const double* r[9];
 };
+// pqpart != nullptr: also p_i . (K4:p_i) of each running right-hand side,
+// per block, at pqpart[i * k4_contract_multi_blocks(V) + block]
 void launch_k4_contract_multi(long V, const double* K4, const Ptr9& pp, const CGScalars* cs,
-                              int nrhs, hipStream_t s, const Ptr9r* rr = nullptr);
+                              int nrhs, hipStream_t s, const Ptr9r* rr = nullptr,
+                              double* pqpart = nullptr);
+int k4_contract_multi_blocks(long V);
 
 // reductions: result -> d_out[0..nres) on device (two-stage, fixed order)
 void launch_dot(const double* x, const double* y, long n, double* partials, double* d_out,
