@@ -36,26 +36,27 @@ import cpfft_amd as C  # noqa: E402
 from cpfft_amd.configs import BATCH_WORKLOADS, WORKLOADS  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# The dominant kernel group is the CG operator of one iteration on the
-# compile-time-planned path: pass A forms the direction p = r + beta p, K4:p
-# and, for the textbook (symmetric projection) operator, the p.q partial sums
-# as p.(K4:p); the five passes apply the projection and pass D writes q.
-# Algorithmic bytes per voxel: read p, r (72 each) and K4 (648), write p and
-# q (72 each).
-CG_OPERATOR_BYTES_PER_VOXEL = 72.0 + 72.0 + 648.0 + 72.0 + 72.0
+# The dominant kernel group is one CG iteration on the compile-time-planned
+# path with the textbook (symmetric projection) operator, gamma_mode 1, which
+# every workload here uses: pass A forms the
+# direction p = r + beta p, K4:p and p.(K4:p) = p.q; a single-block kernel
+# forms alpha; the three middle passes apply the projection; pass D applies
+# x += alpha p and r -= alpha q without storing q and writes the r.r partial
+# sums (only the single-block convergence test is outside the group).
+# Algorithmic bytes per voxel: read K4 (648), p, r and x, write p, r and x
+# (72 each).
+CG_ITERATION_BYTES_PER_VOXEL = 648.0 + 6 * 72.0
 
 
-def cg_operator_traffic_per_voxel(N, gamma_mode):
+def cg_iteration_traffic_per_voxel(N):
     """HBM bytes per voxel the five passes move for that group: pass A reads
     r, p and K4 and writes p and the half spectrum; eight passes over the
-    nine-component half spectrum; pass D writes q (and, for the reference's
-    even-N operator, reads p again for p.q).  Without the CG work the same
-    count gives 1377 B/voxel for G_K_dF with K4 at 128^3, which
-    FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel) reproduce
-    to 0.1%."""
+    nine-component half spectrum; pass D reads r, x and p and writes r and
+    x.  The same count gives 1377 B/voxel for G_K_dF with K4 alone at 128^3,
+    which FETCH_SIZE/WRITE_SIZE counters (calibrated on a copy kernel)
+    reproduce to 0.1%."""
     nz = N // 2 + 1
-    reread = 0.0 if gamma_mode == 1 else 72.0
-    return 72.0 * 2 + reread + 648.0 + 72.0 * 2 + 8.0 * 144.0 * nz / N
+    return 72.0 * 2 + 648.0 + 72.0 + 72.0 * 3 + 72.0 * 2 + 8.0 * 144.0 * nz / N
 
 
 def device_sync():
@@ -293,7 +294,7 @@ def main():
     n_gkdf = d[0]
     steps_per_s = K / wall
     gk_ms = avg[0]
-    achieved = CG_OPERATOR_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
+    achieved = CG_ITERATION_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
     out = {
         "metric": "converged load steps/s at 128^3 (plain method; warm start in warm_start)",
         "value": steps_per_s,
@@ -322,18 +323,18 @@ def main():
             "local_failures": d[6],
         },
         "category_s": {"pcg": float(times[0]), "sig_eps": float(times[1]), "total": float(times[2])},
-        "kernel_ms": {"cg_operator": avg[0], "drive_eps_sig": avg[1], "cg_update": avg[2],
+        "kernel_ms": {"cg_iteration_group": avg[0], "drive_eps_sig": avg[1], "cg_check": avg[2],
                       "G_K_dF_no_K4": avg[3]},
         "roofline": {
-            "kernel": ("CG operator: p = r + beta p, G_K_dF with K4, p.q in pass A "
-                       "(5-kernel launch group)"),
+            "kernel": ("CG iteration: p = r + beta p and G_K_dF with K4 and p.q (pass A), "
+                       "alpha, projection, x and r update with r.r (pass D); 6-kernel launch group"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": cg_operator_traffic_per_voxel(wl.N, wl.gamma_mode) * V,
-            "algorithmic_bytes_per_launch": CG_OPERATOR_BYTES_PER_VOXEL * V,
+            "traffic": cg_iteration_traffic_per_voxel(wl.N) * V,
+            "algorithmic_bytes_per_launch": CG_ITERATION_BYTES_PER_VOXEL * V,
             "avg_launch_ms": gk_ms,
             "launches_measured": int(nlaunch[0]),
         },

@@ -53,7 +53,8 @@ struct cpfft_ctx {
   double* tmp9 = nullptr;  // tangent_homo right-hand side
   double* tan6 = nullptr;  // per-voxel material tangent [36][V]
   double* partials = nullptr;
-  double* dpart = nullptr;  // per-block p.q partial sums of the fused pass D
+  double* dpart = nullptr;  // p.q partial sums of the fused passes A or D
+  double* rpart = nullptr;  // r.r partial sums of pass D in its update mode
   double* dres = nullptr;  // device results of reductions (16 doubles)
   double* hres = nullptr;  // pinned host mirror
   CGScalars* dcg = nullptr;  // device scalars of the running CG solve
@@ -119,7 +120,7 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (int f = 0; f < CPFFT_NFIELDS; ++f)
     if (c->fields[f]) hipFree(c->fields[f]);
-  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dpart, c->dres, c->orients, c->oid, c->dcnt, c->dcg};
+  void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dpart, c->rpart, c->dres, c->orients, c->oid, c->dcnt, c->dcg};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
@@ -240,6 +241,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   // p.q partial sums: pass D's (gkdf_dot_parts) or pass A's (N*N/2, one per
   // pencil pair)
   HIP_OK(hipMalloc(&c->dpart, std::max({1, gkdf_dot_parts(N), N * N}) * sizeof(double)));
+  HIP_OK(hipMalloc(&c->rpart, std::max(1, gkdf_dot_parts(N)) * sizeof(double)));
   HIP_OK(hipMalloc(&c->dres, 16 * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcg, sizeof(CGScalars)));
@@ -459,13 +461,22 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
   g.skip = &c->dcg->done;
   const bool fused = gkdf_pencil_supported(c->N);
   const int ndp = gkdf_dot_parts(c->N);
-  // the textbook operator is a symmetric projection: pass A forms p.(K4:p),
-  // which equals p.q; the reference's even-N operator is not, and pass D
-  // forms p.q there
+  // The textbook operator is a symmetric projection and p lies in its
+  // range, so p.q = p.(K4:p): pass A forms it, alpha follows pass A, and
+  // pass D applies x += alpha p, r -= alpha q without storing q.  The
+  // reference's even-N operator is not a projection: there pass D forms p.q
+  // and a separate kernel updates x and r.
   const bool apq = fused && c->geom.gamma_mode == 1;
-  const int npq = apq ? c->N * c->N / 2 : ndp;
-  const CGFuse fz{c->p, c->r, c->dcg, apq ? nullptr : c->p, apq ? nullptr : c->dpart, nullptr,
-                  apq ? c->dpart : nullptr};
+  CGFuse fz{c->p, c->r, c->dcg, c->p, c->dpart, nullptr, nullptr, 0, nullptr, nullptr, nullptr};
+  if (apq) {
+    fz.dot_p = nullptr;
+    fz.dot_part = nullptr;
+    fz.pq_part = c->dpart;
+    fz.alpha_np = c->N * c->N / 2;
+    fz.upd_r = c->r;
+    fz.upd_x = x;
+    fz.rr_part = c->rpart;
+  }
   for (int j = 0; j < kChunk; ++j) {
     tstart(c, 0);
     if (fused) {
@@ -475,8 +486,10 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
     }
     tstop(c, 0);
     tstart(c, 2);
-    if (fused) {
-      launch_cg_alpha(c->dpart, npq, c->dcg, c->stream);
+    if (apq) {
+      launch_cg_check(c->rpart, ndp, c->dcg, c->stream);
+    } else if (fused) {
+      launch_cg_alpha(c->dpart, ndp, c->dcg, c->stream);
       launch_cg_update_check(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
     } else {
       launch_cg_tail(n, c->dcg, c->p, c->q, x, c->r, c->partials, c->stream);
@@ -772,8 +785,9 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
   }
   const bool fused = gkdf_pencil_supported(c->N);
   const int ndp = gkdf_dot_parts(c->N);
-  // textbook operator: p_i . q_i = p_i . (K4:p_i) from the K4 pass (see
-  // enqueue_pcg_chunk); the reference's even-N operator: from pass D
+  // textbook operator: p_i . q_i = p_i . (K4:p_i) from the K4 pass, alpha
+  // before the operator, pass D updates r_i without storing q_i (see
+  // enqueue_pcg_chunk); the reference's even-N operator: p.q from pass D
   const bool apq = fused && c->geom.gamma_mode == 1;
   const int nb = k4_contract_multi_blocks(V);
   for (int j = 0; j < kChunk; ++j) {
@@ -784,19 +798,25 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
       g.skip = &c->dcg9[i].done;
       // in place: pass A reads all of Q[i] before pass D writes it
       if (fused) {
-        // pass C: voxel sums of K4:p_i; alpha accumulates mean(K4:x_i), so
-        // the update leaves x_i alone
-        const CGFuse fz{nullptr, nullptr, c->dcg9 + i, apq ? nullptr : Pd[i], apq ? nullptr : c->dpart,
-                        c->hdc + 9 * i, nullptr};
-        launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
-        if (apq)
-          launch_cg_alpha(c->hpq + (long)i * nb, nb, c->dcg9 + i, c->stream, c->hdc + 9 * i,
-                          c->hm + 9 * i, 1.0 / (double)V);
-        else
-          launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream, c->hdc + 9 * i, c->hm + 9 * i,
-                          1.0 / (double)V);
-        launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream,
-                               false);
+        // pass C: voxel sums of K4:p_i, from which the check accumulates
+        // mean(K4:x_i); x_i itself is never formed
+        double* const dc = c->hdc + 9 * i;
+        double* const m = c->hm + 9 * i;
+        const double sc = 1.0 / (double)V;
+        if (apq) {
+          launch_cg_alpha(c->hpq + (long)i * nb, nb, c->dcg9 + i, c->stream);
+          const CGFuse fz{nullptr, nullptr, c->dcg9 + i, nullptr, nullptr, dc, nullptr, 0,
+                          R[i], nullptr, c->rpart};
+          launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
+          launch_cg_check(c->rpart, ndp, c->dcg9 + i, c->stream, dc, m, sc);
+        } else {
+          const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart, dc, nullptr, 0,
+                          nullptr, nullptr, nullptr};
+          launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
+          launch_cg_alpha(c->dpart, ndp, c->dcg9 + i, c->stream);
+          launch_cg_update_check(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream,
+                                 false, dc, m, sc);
+        }
       } else {
         launch_gkdf(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream);
         launch_cg_tail(n, c->dcg9 + i, Pd[i], Q[i], X[i], R[i], c->partials, c->stream);

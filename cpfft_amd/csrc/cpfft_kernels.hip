@@ -472,18 +472,11 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_s(const double* __restrict_
 }
 // alpha = rr / (p.q) from the partial sums (same order as k_sum_parts)
 __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ part, int np,
-                                                     CGScalars* __restrict__ cs,
-                                                     const double* __restrict__ dc,
-                                                     double* __restrict__ m, double scale) {
+                                                     CGScalars* __restrict__ cs) {
   if (cs->done) return;
   __shared__ double sh[kBlock];
   const double pq = ordered_sum<kBlock>(part, np, sh);
-  if (threadIdx.x == 0) {
-    const double a = cs->rr / pq;
-    cs->alpha = a;
-    if (m)
-      for (int c = 0; c < 9; ++c) m[c] += a * (dc[c] * scale);
-  }
+  if (threadIdx.x == 0) cs->alpha = cs->rr / pq;
 }
 
 template <bool XUPD>
@@ -530,13 +523,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars*
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// |r| test from the r.r partial sums (FFT_nr3.f:304-306)
+// |r| test from the r.r partial sums (FFT_nr3.f:304-306); with m, first
+// m[c] += alpha * dc[c] * scale (this iteration's alpha)
 __global__ __launch_bounds__(kBlock) void k_cg_check(const double* __restrict__ part, int np,
-                                                     CGScalars* __restrict__ cs) {
+                                                     CGScalars* __restrict__ cs,
+                                                     const double* __restrict__ dc,
+                                                     double* __restrict__ m, double scale) {
   if (cs->done) return;
   __shared__ double sh[kBlock];
   const double rr1 = ordered_sum<kBlock>(part, np, sh);
-  if (threadIdx.x == 0) cg_check(cs, rr1);
+  if (threadIdx.x == 0) {
+    if (m)
+      for (int c = 0; c < 9; ++c) m[c] += cs->alpha * (dc[c] * scale);
+    cg_check(cs, rr1);
+  }
 }
 __global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const double* __restrict__ r,
                            double* __restrict__ p) {
@@ -547,24 +547,28 @@ __global__ void k_cg_dir_s(long n, const CGScalars* __restrict__ cs, const doubl
 }
 
 void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
-                            double* r, double* partials, hipStream_t s, bool xupd) {
+                            double* r, double* partials, hipStream_t s, bool xupd,
+                            const double* dc, double* m, double scale) {
   if (xupd)
     k_cg_update_s<true><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
   else
     k_cg_update_s<false><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
-  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, dc, m, scale);
 }
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s, const double* dc,
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s) {
+  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs);
+}
+void launch_cg_check(const double* part, int np, CGScalars* cs, hipStream_t s, const double* dc,
                      double* m, double scale) {
-  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs, dc, m, scale);
+  k_cg_check<<<1, kBlock, 0, s>>>(part, np, cs, dc, m, scale);
 }
 
 void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,
                     double* r, double* partials, hipStream_t s) {
   k_dot_part_s<<<kRedBlocks, kBlock, 0, s>>>(p, q, n, partials, cs);
-  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, nullptr, nullptr, 0.0);
+  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
   k_cg_update_s<true><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
-  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, nullptr, nullptr, 0.0);
 }
 void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s) {
   k_cg_dir_s<<<grid_for(n), kBlock, 0, s>>>(n, cs, r, p);

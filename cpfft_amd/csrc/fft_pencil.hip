@@ -442,16 +442,23 @@ __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __
 
 // ---- pass D: complex-to-real FFT along z, two pencils per FFT --------------
 // grid (N/TY, N, 9): blockIdx.x = y tile, blockIdx.y = x, blockIdx.z = c
-// DOT: also writes the block's partial sum of Pd . out to part[block].
-template <int N, bool DOT>
+// MODE 0: out = the transform.  MODE 1 (dot): also the block's partial sum
+// of Pd . out in part[block].  MODE 2 (update, a CG iteration whose alpha is
+// already in cs): the transform q is not stored; instead R -= alpha q, with
+// X != nullptr also X += alpha Pd, and the block's partial sum of the new
+// R . R goes to part[block].
+template <int N, int MODE>
 __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, double* __restrict__ out,
                                               const cplx* __restrict__ twg,
                                               const int* __restrict__ skip,
                                               const double* __restrict__ Pd,
-                                              double* __restrict__ part) {
+                                              double* __restrict__ part, double* __restrict__ R,
+                                              double* __restrict__ X,
+                                              const CGScalars* __restrict__ cs) {
   if (skip && *skip) return;
   constexpr int TY = tile_z(N), P = TY * N, LP = TY / 2, LS = lstride(N), NZ = N / 2 + 1;
   constexpr long V = (long)N * N * N;
+  constexpr bool DOT = MODE == 1, UPD = MODE == 2;
   __shared__ cplx tw[N];
   __shared__ cplx buf[LP * LS];
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
@@ -466,20 +473,26 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
     if (k >= 1 && N - k >= NZ) line[pidx(N - k)] = cplx{A.re + B.im, B.re - A.im};
   }
   const long e0 = ((long)x * N + y0) * N;
-  // DOT: this thread's direction values, loaded before the FFT so that
-  // their latency overlaps it
+  // this thread's vector values, loaded before the FFT so that their
+  // latency overlaps it
   constexpr int PP = (P + kT - 1) / kT;
-  double pv[DOT ? PP : 1];
-  if constexpr (DOT) {
+  double pv[DOT || UPD ? PP : 1], rv[UPD ? PP : 1], xv[UPD ? PP : 1];
+  const bool xu = UPD && X != nullptr;
 #pragma unroll
-    for (int j = 0; j < PP; ++j) {
-      const int p = threadIdx.x + j * kT;
-      pv[j] = p < P ? Pd[c * V + e0 + p] : 0.0;
+  for (int j = 0; j < PP; ++j) {
+    const int p = threadIdx.x + j * kT;
+    const long i = c * V + e0 + p;
+    if constexpr (DOT) pv[j] = p < P ? Pd[i] : 0.0;
+    if constexpr (UPD) {
+      rv[j] = p < P ? R[i] : 0.0;
+      pv[j] = xu && p < P ? Pd[i] : 0.0;
+      xv[j] = xu && p < P ? X[i] : 0.0;
     }
   }
   __syncthreads();
   fft<N, LP, false>(buf, tw);
   const double sc = 1.0 / (double)V;
+  const double a = UPD ? cs->alpha : 0.0;
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < PP; ++j) {
@@ -488,11 +501,19 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
       const int yy = p / N, z = p - yy * N;
       const cplx v = buf[(yy >> 1) * LS + pidx(z)];
       const double q = ((yy & 1) ? v.im : v.re) * sc;
-      out[c * V + e0 + p] = q;
-      if constexpr (DOT) acc += q * pv[j];
+      const long i = c * V + e0 + p;
+      if constexpr (UPD) {
+        if (xu) X[i] = xv[j] + a * pv[j];
+        const double rn = rv[j] - a * q;
+        R[i] = rn;
+        acc += rn * rn;
+      } else {
+        out[i] = q;
+        if constexpr (DOT) acc += q * pv[j];
+      }
     }
   }
-  if constexpr (DOT) {
+  if constexpr (DOT || UPD) {
     __shared__ double red[kT];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -586,13 +607,19 @@ void launch(const SpecGeom& g, const double* F, const double* K4, double* out, c
   const int* sk = g.skip;
   const PencilTune& t = g.tune;
   pass_a<N>(K4 ? t.a : t.a0, F, K4, T1, tw, sk, s, fz);
+  if (fz && fz->alpha_np > 0) launch_cg_alpha(fz->pq_part, fz->alpha_np, fz->cs, s);
   pass_b<N>(t.b, true, T1, T2, tw, sk, s);
   pass_c<N>(t.c, T2, tw, g.gamma_mode, sk, s, fz ? fz->dc : nullptr);
   pass_b<N>(t.b, false, T2, T1, tw, sk, s);
-  if (fz && fz->dot_p)
-    kD_zinv<N, true><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, fz->dot_p, fz->dot_part);
+  const dim3 gd(N / TY, N, 9);
+  if (fz && fz->upd_r)
+    kD_zinv<N, 2><<<gd, kT, 0, s>>>(T1, nullptr, tw, sk, fz->upd_x ? fz->dir_p : nullptr,
+                                    fz->rr_part, fz->upd_r, fz->upd_x, fz->cs);
+  else if (fz && fz->dot_p)
+    kD_zinv<N, 1><<<gd, kT, 0, s>>>(T1, out, tw, sk, fz->dot_p, fz->dot_part, nullptr, nullptr,
+                                    nullptr);
   else
-    kD_zinv<N, false><<<dim3(N / TY, N, 9), kT, 0, s>>>(T1, out, tw, sk, nullptr, nullptr);
+    kD_zinv<N, 0><<<gd, kT, 0, s>>>(T1, out, tw, sk, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 // candidate value v of pass p (0 A with K4, 1 A without, 2 B, 3 C), 0 if none

@@ -48,6 +48,12 @@ struct SpecGeom {
 //   symmetric projection G (the textbook operator) and p in its range, as
 //   every CG direction of a right-hand side G(.) is, p . G(K4:p) equals
 //   p . (K4:p), so pass D then needs no dot product.
+// alpha_np > 0: right after pass A, alpha = cs->rr / (sum of the alpha_np
+//   partial sums in pq_part);
+// upd_r != nullptr (alpha already in cs): pass D does not store the
+//   transform q but updates upd_r -= alpha q, with upd_x != nullptr also
+//   upd_x += alpha dir_p, and writes per-block partial sums of the new
+//   upd_r . upd_r into rr_part (gkdf_dot_parts(N) values).
 struct CGFuse {
   double* dir_p;
   const double* dir_r;
@@ -56,6 +62,10 @@ struct CGFuse {
   double* dot_part;
   double* dc;
   double* pq_part;
+  int alpha_np;
+  double* upd_r;
+  double* upd_x;
+  double* rr_part;
 };
 
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
@@ -73,16 +83,20 @@ int pencil_candidate(int N, int pass, int variant);
 bool launch_pencil_pass(const SpecGeom& g, int pass, int variant, const double* F,
                         const double* K4, cplx* spec, const cplx* tw, hipStream_t s);
 
-// alpha = rr / sum(part[0..np)) (the p.q partial sums of a fused pass D);
-// with m != nullptr also m[c] += alpha * dc[c] * scale for c < 9 (the
-// running sum over iterations of alpha times the mean of K4:p)
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s,
+// alpha = rr / sum(part[0..np)) (p.q partial sums)
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s);
+// convergence test and beta from the r.r partial sums part[0..np); with
+// m != nullptr also m[c] += alpha * dc[c] * scale for c < 9 (the running sum
+// over iterations of alpha times the mean of K4:p)
+void launch_cg_check(const double* part, int np, CGScalars* cs, hipStream_t s,
                      const double* dc = nullptr, double* m = nullptr, double scale = 0.0);
 // x += alpha p (xupd; without it x and p are not touched); r -= alpha q
 // with the r.r partial sums, then the convergence test and beta (no p.q, no
 // direction update)
 void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double* q, double* x,
-                            double* r, double* partials, hipStream_t s, bool xupd = true);
+                            double* r, double* partials, hipStream_t s, bool xupd = true,
+                            const double* dc = nullptr, double* m = nullptr,
+                            double scale = 0.0);
 
 // One CG iteration after q = A p: alpha = rr / (p.q); x += alpha p;
 // r -= alpha q; convergence test on |r|; p = r + beta p.  All scalars stay

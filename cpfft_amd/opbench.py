@@ -9,9 +9,10 @@ Operators (the reference routines they replace):
   drive_eps_sig      drive_eps_sig.f:16-45 (Newton iteration 1 of step 1)
   dcopy              a plain copy of one [9][V] field, the bandwidth yardstick
                      for hardware-counter calibration
-  fftPcg iteration   FFT_nr3.f:214-360 (with --cg): the CG operator, which
-                     also forms p = r + beta p and the p.q partial sums, and
-                     the alpha / x, r update / residual-check kernels
+  fftPcg iteration   FFT_nr3.f:214-360 (with --cg): the CG iteration group
+                     (textbook operator: p = r + beta p, K4:p and p.q in pass
+                     A, alpha, projection, x and r update in pass D) and the
+                     residual check
   tangent_homo       tangent_homo.f:11-73 (with --homo): the nine CG solves
                      run together, host wall time of one call
   warm-start network (with --mlp; no counterpart in the reference): forward
@@ -19,8 +20,8 @@ Operators (the reference routines they replace):
                      random rows, host wall time per call
 
 Prints one JSON line with the average ms and the algorithmic HBM GB/s of each
-(bytes per voxel: 792 with K4, 144 without, 144 for the copy, 936 for the
-CG operator: read p, r, K4, write p, q).
+(bytes per voxel: 792 with K4, 144 without, 144 for the copy, 1080 for the
+CG iteration group: read K4, p, r, x, write p, r, x).
 """
 import argparse
 import json
@@ -34,7 +35,7 @@ from .configs import WORKLOADS
 BYTES_GKDF_K4 = 792.0   # read dF 72 + K4 648, write 72
 BYTES_GKDF = 144.0      # read 72, write 72
 BYTES_COPY = 144.0
-BYTES_CG_OP = 936.0     # read p 72, r 72, K4 648, write p 72, q 72
+BYTES_CG_IT = 1080.0    # read K4 648, p, r, x 72 each, write p, r, x 72 each
 
 
 def main():
@@ -91,8 +92,8 @@ def main():
         s.sync()
         avg, n = s.kernel_times()
         res.update({"cg_rc": rc, "cg_iters": it, "cg_relres": rel,
-                    "cg_operator_ms": avg[0], "cg_operator_GBs": BYTES_CG_OP * V / avg[0] * 1e-6,
-                    "cg_update_ms": avg[2], "cg_iter_ms": avg[0] + avg[2]})
+                    "cg_group_ms": avg[0], "cg_group_GBs": BYTES_CG_IT * V / avg[0] * 1e-6,
+                    "cg_check_ms": avg[2], "cg_iter_ms": avg[0] + avg[2]})
     if args.homo:
         c0 = s.counters()
         t0 = time.perf_counter()
