@@ -46,11 +46,14 @@ def main():
     ap.add_argument("--cg", action="store_true", help="also time one CG solve")
     ap.add_argument("--homo", action="store_true", help="also time one tangent_homo call")
     ap.add_argument("--mlp", action="store_true", help="also time the warm-start network")
+    ap.add_argument("--tune", default="", help="FFT launch shapes a,a0,b,c (default: untuned)")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     V = wl.N ** 3
     s = Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
-    print(f"[opbench] {wl.name} N={wl.N} reps={args.reps}", flush=True)
+    if args.tune:
+        s.set_tune([int(v) for v in args.tune.split(",")])
+    print(f"[opbench] {wl.name} N={wl.N} reps={args.reps} shapes {s.tune()}", flush=True)
     # K4 of the elastic state; a smooth strain field as the operand
     s.drive_eps_sig(1, 0)
     rng = np.random.default_rng(1)
@@ -77,7 +80,7 @@ def main():
     s.sync()
     cp = (time.perf_counter() - t0) / args.reps * 1e3
     res = {
-        "workload": wl.name, "N": wl.N, "reps": args.reps,
+        "workload": wl.name, "N": wl.N, "reps": args.reps, "shapes": s.tune(),
         "gkdf_k4_ms": gk, "gkdf_k4_GBs": BYTES_GKDF_K4 * V / gk * 1e-6,
         "gkdf_ms": gk0, "gkdf_GBs": BYTES_GKDF * V / gk0 * 1e-6,
         "dcopy_ms_wall": cp, "dcopy_GBs_wall": BYTES_COPY * V / cp * 1e-6,
