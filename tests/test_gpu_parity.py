@@ -127,10 +127,16 @@ def test_drive_eps_sig(jac_exact):
     s.close()
 
 
-@pytest.mark.parametrize("N,mode", [(7, 0), (8, 1), (16, 1)])
-def test_pcg_and_tangent_homo(N, mode):
+@pytest.mark.parametrize("N,mode,ngr", [(7, 0, 4), (8, 1, 4), (16, 0, 1), (16, 1, 4)])
+def test_pcg_and_tangent_homo(N, mode, ngr):
+    """N = 16 runs the planned FFT path in both operator modes.  With the
+    reference's even-N operator (mode 0, not a projection) CG on
+    tangent_homo's right-hand sides of a polycrystal does not converge within
+    1000 iterations for some of them, in the oracle as on the GPU (4 of 9 for
+    the 4-grain 16^3 case; the reference aborts there, FFT_nr3.f:335-339), so
+    that mode is checked on a single crystal, where each solve converges."""
     V = N ** 3
-    s, ang = solver(N, mode)
+    s, ang = solver(N, mode, ngr=ngr)
     oc = O.crystal()
     I = np.zeros((9, V))
     I[[0, 4, 8]] = 1.0
@@ -151,10 +157,12 @@ def test_pcg_and_tangent_homo(N, mode):
     s.close()
 
 
-@pytest.mark.parametrize("N,mode,nstep", [(7, 0, 3), (8, 1, 3), (16, 1, 2)])
-def test_load_steps(N, mode, nstep):
-    """The Fortran driver over the C-ABI against the oracle's FFT_nr3 loop."""
-    wl = C.Workload("t", N, 4, 20261202 + 99, nstep, 0.0005 * nstep, gamma_mode=mode)
+@pytest.mark.parametrize("N,mode,nstep,ngr", [(7, 0, 3, 4), (8, 1, 3, 4), (16, 0, 2, 1),
+                                              (16, 1, 2, 4)])
+def test_load_steps(N, mode, nstep, ngr):
+    """The Fortran driver over the C-ABI against the oracle's FFT_nr3 loop
+    (mode 0 at N = 16 on a single crystal, as in test_pcg_and_tangent_homo)."""
+    wl = C.Workload("t", N, ngr, 20261202 + 99, nstep, 0.0005 * nstep, gamma_mode=mode)
     ang = wl.angles()
     BC, isNBC = wl.bc()
     s = C.Solver(N, ang, wl.material, dt=wl.dt, gamma_mode=mode)
