@@ -14,10 +14,11 @@ are reported beside the headline.
 With --workload c4_batch the run is the batched uneven-RVE workload
 instead (24 RVEs of edge 32-96 solved at once; see batch_bench).
 
-Prints one JSON line.  The roofline object describes G_K_dF with K4 (the
-CG matrix-vector product, five kernels launched as one group), whose device
-time is measured with HIP events on the solver's stream inside the timed
-region.  The cpu_baseline object times the oracle (the CPU restatement, all
+Prints one JSON line.  The roofline object describes one CG iteration
+(fftPcg): the direction update, the G_K_dF with K4 of the direction and the
+x, r update, launched as one group of six kernels (pass A, alpha, passes B,
+C, B', pass D); its device time is measured with HIP events on the solver's
+stream inside the timed region.  The cpu_baseline object times the oracle (the CPU restatement, all
 host cores) on a bounded sample of the same workload and scales it by this
 run's operator counts per load step.
 """
