@@ -113,6 +113,9 @@ int cpfft_gkdf(cpfft_ctx* ctx, const double* F, double* GKF, int flgK);
 
 /* fftPcg (FFT_nr3.f:214-360): x = CG solution of Gamma:(K4:x) = b, x0 = 0,
  * stop at ||r|| <= tol*||b|| or ||r|| <= tol, 1000 iterations at most.
+ * b must lie in the range of Gamma (b = Gamma:(.)), as every right-hand
+ * side FFT_nr3 and tangent_homo form does: with the textbook operator (a
+ * symmetric projection) p.q is then formed as p.(K4:p).
  * Returns 0, 1 (zero right-hand side, x = 0), -1 (tol outside (eps,1)),
  * -2 (no convergence).                                                    */
 int cpfft_pcg(cpfft_ctx* ctx, const double* b, double* x, double tol, int* iters,
