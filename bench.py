@@ -37,6 +37,9 @@ import cpfft_amd as C  # noqa: E402
 from cpfft_amd.configs import BATCH_WORKLOADS, WORKLOADS  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# timed steps record HIP events on every TIMING_STRIDE-th launch group of a
+# family (coprime to the CG chunk of eight iterations)
+TIMING_STRIDE = 7
 # The dominant kernel group is one CG iteration on the compile-time-planned
 # path with the textbook (symmetric projection) operator, gamma_mode 1, which
 # every workload here uses: pass A forms the
@@ -117,7 +120,10 @@ def timed_steps(s, wl, BC_all, isNBC, W, K):
     """Time load steps W+1..W+K on a solver that has run steps 1..W;
     returns (wall, rc, Pbar, cnt, times, counter deltas, kernel avg ms, counts)."""
     c0 = s.counters()
-    s.set_timing(True)
+    # HIP events on every 7th launch group of each family: each recorded
+    # pair idles the stream for about 10 us, so timing every group would
+    # slow the steps being timed
+    s.set_timing(TIMING_STRIDE)
     device_sync()
     s.sync()
     t0 = time.perf_counter()

@@ -285,7 +285,8 @@ class Solver:
         _check(lib().cpfft_set_tune(self.h, arr), "cpfft_set_tune")
 
     def set_timing(self, on):
-        lib().cpfft_set_timing(self.h, int(on))
+        """0/False off; True or 1 every launch group; k > 1 every k-th."""
+        _check(lib().cpfft_set_timing(self.h, int(on)), "cpfft_set_timing")
 
     def kernel_times(self):
         """(average ms per launch group, counts) for: G_K_dF with K4,

@@ -89,10 +89,15 @@ struct cpfft_ctx {
   std::vector<std::pair<const double*, hipGraphExec_t>> pcg_graphs;
   hipGraphExec_t homo_graph = nullptr;
   // kernel timing: event pairs recorded on the stream without host
-  // synchronisation, resolved by cpfft_kernel_times
+  // synchronisation, resolved by cpfft_kernel_times; timing = k > 0 records
+  // every k-th launch group of each family (ev_seq counts them), ev_it the
+  // CG iteration a recorded group belongs to (ev_cur, -1 outside cpfft_pcg)
   static constexpr int kFam = 4, kRing = 8192;
   int timing = 0;
+  long ev_seq[kFam] = {0, 0, 0, 0};
+  int ev_cur = -1;
   std::vector<hipEvent_t> ev[kFam][2];
+  std::vector<int> ev_it[kFam];
   int ev_n[kFam] = {0, 0, 0, 0};
   double t_ms[kFam] = {0, 0, 0, 0};
   long t_n[kFam] = {0, 0, 0, 0};
@@ -117,12 +122,12 @@ extern "C" const char* cpfft_last_error(void) { return g_err.c_str(); }
 
 extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (!c) return 0;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int f = 0; f < CPFFT_NFIELDS; ++f)
-    if (c->fields[f]) hipFree(c->fields[f]);
+    if (c->fields[f]) (void)hipFree(c->fields[f]);
   void* bufs[] = {c->spec, c->tw, c->r, c->p, c->q, c->tmp9, c->tan6, c->partials, c->dpart, c->rpart, c->dres, c->orients, c->oid, c->dcnt, c->dcg};
   for (void* b : bufs)
-    if (b) hipFree(b);
+    if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
   if (c->hcg) (void)hipHostFree(c->hcg);
   if (c->hcg9) (void)hipHostFree(c->hcg9);
@@ -143,9 +148,9 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   }
   for (int f = 0; f < cpfft_ctx::kFam; ++f)
     for (int k = 0; k < 2; ++k)
-      for (hipEvent_t e : c->ev[f][k]) hipEventDestroy(e);
+      for (hipEvent_t e : c->ev[f][k]) (void)hipEventDestroy(e);
   drop_graphs(c);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
@@ -346,6 +351,7 @@ static void flush_events(cpfft_ctx* c) {
 }
 static void tstart(cpfft_ctx* c, int fam) {
   if (!c->timing) return;
+  if (c->ev_seq[fam]++ % c->timing != 0) return;  // not sampled: tstop does nothing
   if (c->ev_n[fam] == cpfft_ctx::kRing) flush_events(c);
   int k = c->ev_n[fam];
   if ((int)c->ev[fam][0].size() <= k) {
@@ -353,8 +359,10 @@ static void tstart(cpfft_ctx* c, int fam) {
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
     c->ev[fam][0].push_back(a);
     c->ev[fam][1].push_back(b);
+    c->ev_it[fam].push_back(-1);
   }
   (void)hipEventRecord(c->ev[fam][0][k], c->stream);
+  c->ev_it[fam][k] = c->ev_cur;
   c->open_fam = fam;
 }
 static void tstop(cpfft_ctx* c, int fam) {
@@ -365,9 +373,15 @@ static void tstop(cpfft_ctx* c, int fam) {
 }
 
 extern "C" int cpfft_set_timing(cpfft_ctx* c, int on) {
+  if (!c) return fail(-1, "cpfft_set_timing: null context");
+  if (on < 0) return fail(-1, "cpfft_set_timing: negative sampling stride");
   flush_events(c);
   c->timing = on;
-  for (int i = 0; i < cpfft_ctx::kFam; ++i) { c->t_ms[i] = 0; c->t_n[i] = 0; }
+  for (int i = 0; i < cpfft_ctx::kFam; ++i) {
+    c->t_ms[i] = 0;
+    c->t_n[i] = 0;
+    c->ev_seq[i] = 0;
+  }
   return 0;
 }
 
@@ -455,7 +469,9 @@ static constexpr int kChunk = 8;
 // alpha, x and r updates, convergence test.  Every kernel reads the scalars
 // in c->dcg and returns at once after convergence, so the chunk has no
 // host-side control flow and can be replayed from a graph.
-static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
+// base: the solve's iteration number of the chunk's first iteration (only
+// labels timing events)
+static void enqueue_pcg_chunk(cpfft_ctx* c, double* x, int base = 0) {
   const long n = 9 * c->V;
   SpecGeom g = c->geom;
   g.skip = &c->dcg->done;
@@ -478,6 +494,7 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
     fz.rr_part = c->rpart;
   }
   for (int j = 0; j < kChunk; ++j) {
+    c->ev_cur = base + j;
     tstart(c, 0);
     if (fused) {
       launch_gkdf_pencil(g, c->p, c->fields[CPFFT_K4], c->q, c->spec, c->tw, c->stream, &fz);
@@ -497,6 +514,7 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x) {
     tstop(c, 2);
     if (!fused) launch_cg_dir_dev(n, c->dcg, c->r, c->p, c->stream);
   }
+  c->ev_cur = -1;
 }
 
 // record what enqueue() launches on the context's stream into an
@@ -569,7 +587,7 @@ extern "C" int cpfft_autotune(cpfft_ctx* c, int reps, int force) {
   for (int pass = 0; pass < 4; ++pass) {
     float best = 0.f;
     int bi = 0;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < kPencilCands; ++v) {
       if (!pencil_candidate(c->N, pass, v)) continue;
       launch_pencil_pass(c->geom, pass, v, F, K4, c->spec, c->tw, c->stream);  // warm-up
       HIP_OK(hipEventRecord(e0, c->stream));
@@ -610,7 +628,7 @@ extern "C" int cpfft_set_tune(cpfft_ctx* c, const int in[4]) {
   int* knob[4] = {&t.a, &t.a0, &t.b, &t.c};
   for (int p = 0; p < 4; ++p) {
     int found = -1;
-    for (int v = 0; v < 3 && found < 0; ++v)
+    for (int v = 0; v < kPencilCands && found < 0; ++v)
       if (pencil_candidate(c->N, p, v) == in[p] && in[p] > 0) found = v;
     if (found < 0) return fail(-1, "cpfft_set_tune: shape is not among the candidates for this N");
     *knob[p] = found;
@@ -692,7 +710,7 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
     } else {
       if (c->timing && (c->ev_n[0] + kChunk > cpfft_ctx::kRing || c->ev_n[2] + kChunk > cpfft_ctx::kRing))
         flush_events(c);
-      enqueue_pcg_chunk(c, x);
+      enqueue_pcg_chunk(c, x, launched);
     }
     launched += kChunk;
     HIP_OK(hipGetLastError());
@@ -701,10 +719,9 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
     if (c->hcg->done) break;
   }
   const int k = c->hcg->k;
-  if (c->timing) {  // drop the events of the no-op iterations at the end
-    c->ev_n[0] -= launched - k;
-    c->ev_n[2] -= launched - k;
-  }
+  if (c->timing)  // drop the events of the no-op iterations at the end
+    for (int f : {0, 2})
+      while (c->ev_n[f] > 0 && c->ev_it[f][c->ev_n[f] - 1] >= k) c->ev_n[f]--;
   c->cg_iters += k;
   c->gkdf_calls += k;
   if (iters) *iters = k;

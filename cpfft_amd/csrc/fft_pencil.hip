@@ -10,6 +10,7 @@
 //   B  y forward:  complex FFT along y, transposing      T1 -> T2[c][kz][y][x]
 //   C  x:          FFT along x of all nine components, Green projection at
 //                  each frequency, inverse FFT along x, in place in T2
+//                  (for N = 64, 128, 256 with the transforms in registers)
 //   B' y inverse:  inverse FFT along y, transposing       T2 -> T1
 //   D  z inverse:  complex-to-real FFT along z (two pencils per complex
 //                  FFT), scale 1/N^3                      T1 -> out[c][x][y][z]
@@ -56,7 +57,7 @@ constexpr int tile_z(int n) {
   return best;
 }
 // Launch-shape candidates of the autotuner (PencilTune): the i-th largest
-// (i = 0, 1, 2) of the admissible values, 0 when there is none.  Every
+// (i = 0, 1, ...) of the admissible values, 0 when there is none.  Every
 // candidate computes bit-identical results: the shapes only change which
 // block transforms which lines, never the arithmetic of a line.
 // pass A: pencils per block, even divisors t of N with t*N <= 1024
@@ -79,12 +80,38 @@ constexpr int ty_cand(int n, int i) {
     }
   return 0;
 }
-// pass C: y lines per block (all nine components of each), 1 or 2 where
-// they fit the 64 KiB of static LDS
-constexpr int xl_cand(int n, int i) {
-  return i < 2 && (9 * (i + 1) * lstride(n) + n) * 16 <= 65536 ? i + 1 : 0;
+// pass C in registers (kC_xr) needs N = 8 M with M a multiple of 8
+constexpr bool regc_ok(int n) {
+  const Plan p = plan_for(n);
+  return p.nst >= 2 && p.r[0] == 8 && (n / 8) % 8 == 0;
 }
-constexpr int kCands = 3;
+constexpr bool lds_fits(int lines, int n) { return (lines * lstride(n) + n) * 16 <= 65536; }
+// pass C: component lines per block, each candidate where its lines fit
+// the 64 KiB of static LDS.  Where regc_ok: kC_xr with the three components
+// of one tensor row for 8, 4 or 2 y lines (24, 12, 6).  Elsewhere: kC_x
+// with all nine components of 2 or 1 y lines (18, 9).  One kernel family
+// per N: the two agree bit for bit at N = 64 but only to the last bits at
+// 128 and 256, where the compiler fuses multiplies and adds differently
+// (measured at 128^3: kC_xr 60-66 us, kC_x 85-88 us).
+constexpr int xl_cand(int n, int i) {
+  int k = 0;
+  if (regc_ok(n)) {
+    for (int yb = 8; yb >= 2; yb /= 2)
+      if (lds_fits(3 * yb, n)) {
+        if (k == i) return 3 * yb;
+        ++k;
+      }
+    return 0;
+  }
+  for (int yb = 2; yb >= 1; --yb)
+    if (lds_fits(9 * yb, n)) {
+      if (k == i) return 9 * yb;
+      ++k;
+    }
+  return 0;
+}
+constexpr bool xl_reg(int v) { return v % 9 != 0; }  // kC_xr (else kC_x)
+static_assert(kPencilCands >= 5, "candidate slots");
 
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
@@ -440,6 +467,112 @@ __global__ __launch_bounds__(kT) void kC_x(cplx* __restrict__ T2, const cplx* __
   }
 }
 
+// ---- pass C with the transforms in registers -----------------------------
+// For N = 8 M with M a multiple of 8 (64, 128, 256).  The Stockham passes
+// of fft() are regrouped without changing their arithmetic: the radix-8
+// first pass combines elements j + M t of column j (t < 8) and writes
+// element 8 j + q; every later pass maps the elements of one residue class
+// q mod 8 onto the same class.  So a thread first transforms columns
+// j = g, g + 8, ... with inputs loaded straight from HBM, the block
+// exchanges once through LDS, and thread q then runs all later passes on
+// the M elements q + 8 u of its class in registers; the results land in
+// natural order.  Per line: eight threads, two LDS exchanges per transform
+// instead of one per pass.
+// the Stockham passes ST.. of fft() on residue class q (v[u] = element
+// q + 8 u), NS the product of the earlier radices
+template <int N, bool FWD, int ST, int NS>
+__device__ __forceinline__ void reg_passes(cplx* v, int q, const cplx* tw) {
+  constexpr Plan P = plan_for(N);
+  if constexpr (ST < P.nst) {
+    constexpr int R1 = P.r[0], M = N / R1, R = P.r[ST], NBT = M / R, m = NS / R1;
+    constexpr int TS = N / (NS * R);
+    cplx w[M];
+#pragma unroll
+    for (int b = 0; b < NBT; ++b) {
+      cplx x[R];
+#pragma unroll
+      for (int t = 0; t < R; ++t) x[t] = v[b + t * NBT];
+      const int k = q + R1 * (b % m);  // the butterfly's j % NS in fft()
+#pragma unroll
+      for (int t = 1; t < R; ++t) x[t] = cmul(x[t], twid<FWD>(tw, t * k * TS));
+      butterfly<R, FWD, N>(x, tw);
+      const int o = (b / m) * m * R + b % m;
+#pragma unroll
+      for (int t = 0; t < R; ++t) w[o + t * m] = x[t];
+    }
+#pragma unroll
+    for (int u = 0; u < M; ++u) v[u] = w[u];
+    reg_passes<N, FWD, ST + 1, NS * R>(v, q, tw);
+  }
+}
+
+// YB consecutive y lines of the three components of one tensor row per
+// block (the projection of row i reads only row i); eight threads per line.
+// grid (N/YB, NZ, 3): blockIdx.x = y / YB, blockIdx.y = kz, blockIdx.z = row.
+// Same operations on the same operands as kC_x.
+template <int N, int YB>
+__global__ __launch_bounds__(24 * YB) void kC_xr(cplx* __restrict__ T2,
+                                                 const cplx* __restrict__ twg, int gamma_mode,
+                                                 const int* __restrict__ skip,
+                                                 double* __restrict__ dc) {
+  if (skip && *skip) return;
+  constexpr int R1 = 8, M = N / R1, G = M / R1, LS = lstride(N), NZ = N / 2 + 1, T = 24 * YB;
+  static_assert(regc_ok(N), "register pass C needs N = 8 M, M a multiple of 8");
+  __shared__ cplx tw[N];
+  __shared__ cplx buf[3 * YB * LS];
+  const int y0 = blockIdx.x * YB, kz = blockIdx.y, row = blockIdx.z;
+  const int L = threadIdx.x / R1, g = threadIdx.x % R1;  // line L = jc * YB + t
+  const int jc = L / YB, t = L - jc * YB, c = 3 * row + jc;
+  cplx* gl = T2 + (((long)c * NZ + kz) * N + y0 + t) * N;
+  cplx* bl = buf + L * LS;
+  load_tw<N, T>(tw, twg);
+  cplx x[G][R1];
+#pragma unroll
+  for (int gc = 0; gc < G; ++gc)
+#pragma unroll
+    for (int u = 0; u < R1; ++u) x[gc][u] = gl[g + R1 * gc + M * u];
+  // forward: first pass on columns g + 8 gc, exchange, later passes
+#pragma unroll
+  for (int gc = 0; gc < G; ++gc) {
+    butterfly<R1, true, N>(x[gc], tw);
+#pragma unroll
+    for (int q = 0; q < R1; ++q) bl[pidx((g + R1 * gc) * R1 + q)] = x[gc][q];
+  }
+  __syncthreads();
+  cplx v[M];
+#pragma unroll
+  for (int u = 0; u < M; ++u) v[u] = bl[pidx(g + R1 * u)];
+  reg_passes<N, true, 1, R1>(v, g, tw);
+  if (dc && blockIdx.x == 0 && kz == 0 && t == 0 && g == 0) dc[c] = v[0].re;
+  // each thread rewrites the elements it read: no barrier before this
+#pragma unroll
+  for (int u = 0; u < M; ++u) bl[pidx(g + R1 * u)] = v[u];
+  __syncthreads();
+  // projection at the inputs of the inverse first pass
+  const cplx* r0 = buf + t * LS;
+#pragma unroll
+  for (int gc = 0; gc < G; ++gc)
+#pragma unroll
+    for (int u = 0; u < R1; ++u) {
+      const int xx = g + R1 * gc + M * u;
+      const cplx A3[3] = {r0[pidx(xx)], r0[YB * LS + pidx(xx)], r0[2 * YB * LS + pidx(xx)]};
+      x[gc][u] = project1(N, gamma_mode, xx, y0 + t, kz, A3, jc);
+    }
+#pragma unroll
+  for (int gc = 0; gc < G; ++gc) butterfly<R1, false, N>(x[gc], tw);
+  __syncthreads();
+#pragma unroll
+  for (int gc = 0; gc < G; ++gc)
+#pragma unroll
+    for (int q = 0; q < R1; ++q) bl[pidx((g + R1 * gc) * R1 + q)] = x[gc][q];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < M; ++u) v[u] = bl[pidx(g + R1 * u)];
+  reg_passes<N, false, 1, R1>(v, g, tw);
+#pragma unroll
+  for (int u = 0; u < M; ++u) gl[g + R1 * u] = v[u];
+}
+
 // ---- pass D: complex-to-real FFT along z, two pencils per FFT --------------
 // grid (N/TY, N, 9): blockIdx.x = y tile, blockIdx.y = x, blockIdx.z = c
 // MODE 0: out = the transform.  MODE 1 (dot): also the block's partial sum
@@ -555,8 +688,13 @@ void launch_b(bool fwd, const cplx* src, cplx* dst, const cplx* tw, const int* s
 }
 template <int N, int I>
 void launch_c(cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipStream_t s, double* dc) {
-  constexpr int YB = xl_cand(N, I), NZ = N / 2 + 1;
-  if constexpr (YB > 0) kC_x<N, YB><<<dim3(N / YB, NZ), kT, 0, s>>>(T2, tw, gamma_mode, sk, dc);
+  constexpr int V = xl_cand(N, I), NZ = N / 2 + 1;
+  if constexpr (V > 0 && xl_reg(V)) {
+    if constexpr (regc_ok(N))
+      kC_xr<N, V / 3><<<dim3(N / (V / 3), NZ, 3), 8 * V, 0, s>>>(T2, tw, gamma_mode, sk, dc);
+  } else if constexpr (V > 0) {
+    kC_x<N, V / 9><<<dim3(N / (V / 9), NZ), kT, 0, s>>>(T2, tw, gamma_mode, sk, dc);
+  }
 }
 template <int N>
 void pass_a(int i, const double* F, const double* K4, cplx* T1, const cplx* tw, const int* sk,
@@ -581,6 +719,9 @@ void pass_c(int i, cplx* T2, const cplx* tw, int gamma_mode, const int* sk, hipS
             double* dc = nullptr) {
   switch (i) {
     case 1: launch_c<N, 1>(T2, tw, gamma_mode, sk, s, dc); break;
+    case 2: launch_c<N, 2>(T2, tw, gamma_mode, sk, s, dc); break;
+    case 3: launch_c<N, 3>(T2, tw, gamma_mode, sk, s, dc); break;
+    case 4: launch_c<N, 4>(T2, tw, gamma_mode, sk, s, dc); break;
     default: launch_c<N, 0>(T2, tw, gamma_mode, sk, s, dc); break;
   }
 }
@@ -628,7 +769,9 @@ int cand(int p, int v) {
   switch (p) {
     case 0: case 1: return v == 0 ? tz_cand(N, 0) : v == 1 ? tz_cand(N, 1) : v == 2 ? tz_cand(N, 2) : 0;
     case 2: return v == 0 ? ty_cand(N, 0) : v == 1 ? ty_cand(N, 1) : v == 2 ? ty_cand(N, 2) : 0;
-    default: return v == 0 ? xl_cand(N, 0) : v == 1 ? xl_cand(N, 1) : 0;
+    default:
+      return v == 0 ? xl_cand(N, 0) : v == 1 ? xl_cand(N, 1) : v == 2 ? xl_cand(N, 2)
+           : v == 3 ? xl_cand(N, 3) : v == 4 ? xl_cand(N, 4) : 0;
   }
 }
 
@@ -661,7 +804,7 @@ bool launch_gkdf_pencil(const SpecGeom& g, const double* F, const double* K4, do
 }
 
 int pencil_candidate(int N, int pass, int variant) {
-  if (pass < 0 || pass > 3 || variant < 0 || variant >= fp::kCands) return 0;
+  if (pass < 0 || pass > 3 || variant < 0 || variant >= kPencilCands) return 0;
   switch (N) {
     case 16: return fp::cand<16>(pass, variant);
     case 32: return fp::cand<32>(pass, variant);

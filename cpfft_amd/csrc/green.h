@@ -26,6 +26,17 @@ __device__ inline double ddot42n_row(const double* K, const double* F) {
   return t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
 }
 
+// (A q) for one row (A3 = A_i1, A_i2, A_i3)
+__device__ __forceinline__ cplx gamma_aq(const double q[3], const cplx A3[3]) {
+  return cplx{A3[0].re * q[0] + A3[1].re * q[1] + A3[2].re * q[2],
+              A3[0].im * q[0] + A3[1].im * q[1] + A3[2].im * q[2]};
+}
+// o += (A q)_i q_j s
+__device__ __forceinline__ void gamma_term(cplx aq, double qj, double s, cplx& o) {
+  o.re += aq.re * qj * s;
+  o.im += aq.im * qj * s;
+}
+
 // out += w * q_j (A q)_i / |q|^2
 __device__ inline void gamma_acc(const double q[3], const cplx A[9], cplx out[9], double w) {
   double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
@@ -33,14 +44,17 @@ __device__ inline void gamma_acc(const double q[3], const cplx A[9], cplx out[9]
   double s = w / qq;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    cplx aq{A[3 * i].re * q[0] + A[3 * i + 1].re * q[1] + A[3 * i + 2].re * q[2],
-            A[3 * i].im * q[0] + A[3 * i + 1].im * q[1] + A[3 * i + 2].im * q[2]};
+    const cplx aq = gamma_aq(q, A + 3 * i);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      out[3 * i + j].re += aq.re * q[j] * s;
-      out[3 * i + j].im += aq.im * q[j] * s;
-    }
+    for (int j = 0; j < 3; ++j) gamma_term(aq, q[j], s, out[3 * i + j]);
   }
+}
+// component (i, j) of the same sum, from row i of A only
+__device__ inline void gamma_acc1(const double q[3], const cplx A3[3], int j, cplx& o, double w) {
+  double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  if (qq <= 1e-10) return;
+  double s = w / qq;
+  gamma_term(gamma_aq(q, A3), j == 0 ? q[0] : j == 1 ? q[1] : q[2], s, o);
 }
 
 __device__ inline int wrapf(int v, int N) {
@@ -74,6 +88,34 @@ __device__ inline void project9(int N, int gamma_mode, int x, int y, int kz, con
       gamma_acc(q, A, out, 1.0);
     }
   }
+}
+
+// component (i, j) of project9's result, from row i of A (A3 = A[3i..3i+2]):
+// the same operations on the same operands, so the same bits
+__device__ inline cplx project1(int N, int gamma_mode, int x, int y, int kz, const cplx A3[3],
+                                int j) {
+  cplx o{0.0, 0.0};
+  const bool even = (N % 2) == 0;
+  const int f[3] = {x <= N / 2 ? x : x - N, y <= N / 2 ? y : y - N, kz};
+  if (gamma_mode == 0 && even) {
+    double q1[3], q2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      q1[a] = (double)wrapf(f[a] - 1, N);
+      q2[a] = (double)wrapf(-f[a] - 1, N);
+    }
+    gamma_acc1(q1, A3, j, o, 0.5);
+    gamma_acc1(q2, A3, j, o, 0.5);
+  } else {
+    const bool nyq = even && (abs(f[0]) == N / 2 || abs(f[1]) == N / 2 || f[2] == N / 2);
+    if (!nyq) {
+      double q[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) q[a] = (double)wrapf(f[a], N);
+      gamma_acc1(q, A3, j, o, 1.0);
+    }
+  }
+  return o;
 }
 
 }  // namespace cpfft

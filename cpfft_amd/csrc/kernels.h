@@ -19,6 +19,7 @@ namespace cpfft {
 struct PencilTune {
   int a = 0, a0 = 0, b = 0, c = 0;
 };
+constexpr int kPencilCands = 5;  // candidate indices per pass are below this
 
 struct SpecGeom {
   int N, Nz;        // grid edge, number of z frequencies kept (N/2+1)

@@ -180,7 +180,11 @@ int cpfft_counters(cpfft_ctx* ctx, long long out[8]);
  * (no host synchronisation while recording).  out[0..3]: average ms per
  * launch group since cpfft_set_timing of [0] G_K_dF with K4 (five kernels),
  * [1] drive_eps_sig, [2] CG vector update, [3] G_K_dF without K4;
- * out[4..7]: number of launch groups measured.                           */
+ * out[4..7]: number of launch groups measured.
+ * cpfft_set_timing(ctx, k): 0 off; k > 0 records every k-th launch group
+ * of each family.  Each recorded pair costs the stream about 10 us of idle
+ * time around it, so a long run is timed with a sampling stride (bench.py
+ * uses 7, coprime to the CG chunk of 8).  Negative k: -1.                  */
 int cpfft_kernel_times(cpfft_ctx* ctx, double out[8]);
 int cpfft_set_timing(cpfft_ctx* ctx, int on);
 

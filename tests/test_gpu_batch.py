@@ -135,11 +135,13 @@ def test_batch_continues_runs():
         s.close()
 
 
-@pytest.mark.parametrize("N", [16, 32, 48, 64, 96])
+@pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128])
 def test_every_tuned_shape_gives_identical_results(N):
     """The autotuner's candidates (cpfft_set_tune accepts exactly those)
-    change only which block transforms which lines: G_K_dF with and without
-    K4 and a fused CG solve are bitwise equal under every one of them."""
+    change only which block transforms which lines, and for N = 64 and 128
+    whether pass C runs its transforms in LDS or in registers: G_K_dF with
+    and without K4 and a fused CG solve are bitwise equal under every one of
+    them."""
     s = _solver(N)
     s.drive_eps_sig(1, 0)
     rng = np.random.default_rng(3 * N)
@@ -181,4 +183,33 @@ def test_every_tuned_shape_gives_identical_results(N):
     print(f"\n[tune] N={N} default {default} autotuned {chosen}", flush=True)
     got = run()
     assert np.array_equal(got[1], ref[1]) and np.array_equal(got[3], ref[3])
+    s.close()
+
+
+def test_sampled_kernel_timing():
+    """cpfft_set_timing(k) records every k-th launch group; the iterates do
+    not depend on it and the no-op iterations after convergence are not
+    counted."""
+    N = 16
+    s = _solver(N)
+    s.drive_eps_sig(1, 0)
+    rng = np.random.default_rng(11)
+    s.upload(C.DFM, 1e-3 * rng.standard_normal((9, N ** 3)))
+    out = []
+    for k in (0, 1, 3):
+        s.gkdf(C.DFM, C.B, 1)
+        s.dscal(-1.0, C.B)
+        s.set_timing(k)
+        rc, it, _ = s.pcg(C.B, C.WORK, 1e-8)
+        avg, cnt = s.kernel_times()
+        print(f"\n[timing] stride {k}: {it} iterations, groups timed {list(cnt)}", flush=True)
+        assert rc == 0
+        want = 0 if k == 0 else -(-it // k)
+        assert cnt[0] == want and cnt[2] == want
+        assert (avg[0] > 0) == (k > 0)
+        out.append((it, s.download(C.WORK)))
+    assert out[0][0] == out[1][0] == out[2][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][1], out[2][1])
+    with pytest.raises(C.CpfftError):
+        s.set_timing(-1)
     s.close()
