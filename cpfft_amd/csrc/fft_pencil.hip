@@ -21,6 +21,12 @@
 // strided butterfly writes fall on distinct LDS banks.
 #include "kernels.h"
 
+// Products and sums are rounded separately in this file (no contraction
+// into FMAs): the compiler otherwise fuses differently in different
+// instantiations of the same transform (e.g. pass B with 2 or 8 lines per
+// block), and every launch shape must give the same bits.
+#pragma clang fp contract(off)
+
 namespace cpfft {
 namespace fp {
 
@@ -90,9 +96,8 @@ constexpr bool lds_fits(int lines, int n) { return (lines * lstride(n) + n) * 16
 // the 64 KiB of static LDS.  Where regc_ok: kC_xr with the three components
 // of one tensor row for 8, 4 or 2 y lines (24, 12, 6).  Elsewhere: kC_x
 // with all nine components of 2 or 1 y lines (18, 9).  One kernel family
-// per N: the two agree bit for bit at N = 64 but only to the last bits at
-// 128 and 256, where the compiler fuses multiplies and adds differently
-// (measured at 128^3: kC_xr 60-66 us, kC_x 85-88 us).
+// per N: kC_xr is the faster wherever it applies (at 128^3 60-66 us against
+// 85-88 us for kC_x).
 constexpr int xl_cand(int n, int i) {
   int k = 0;
   if (regc_ok(n)) {

@@ -26,13 +26,17 @@ __device__ inline double ddot42n_row(const double* K, const double* F) {
   return t[0] + (((t[1] + t[5]) + (t[3] + t[7])) + ((t[2] + t[6]) + (t[4] + t[8])));
 }
 
-// (A q) for one row (A3 = A_i1, A_i2, A_i3)
+// (A q) for one row (A3 = A_i1, A_i2, A_i3).  Here and in gamma_term
+// products are rounded before the sums, so every kernel that projects gets
+// the same bits.
 __device__ __forceinline__ cplx gamma_aq(const double q[3], const cplx A3[3]) {
+#pragma clang fp contract(off)
   return cplx{A3[0].re * q[0] + A3[1].re * q[1] + A3[2].re * q[2],
               A3[0].im * q[0] + A3[1].im * q[1] + A3[2].im * q[2]};
 }
 // o += (A q)_i q_j s
 __device__ __forceinline__ void gamma_term(cplx aq, double qj, double s, cplx& o) {
+#pragma clang fp contract(off)
   o.re += aq.re * qj * s;
   o.im += aq.im * qj * s;
 }
