@@ -42,7 +42,7 @@ EXPORTS = [
     "cpfft_upload", "cpfft_download", "cpfft_reset", "cpfft_sync", "cpfft_driver_state",
     "cpfft_gkdf", "cpfft_pcg",
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
-    "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_counters",
+    "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_f2disp", "cpfft_counters",
     "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_autotune",
     "cpfft_get_tune", "cpfft_set_tune",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
@@ -107,6 +107,7 @@ def lib():
         L.cpfft_dcopy.argtypes = [_vp, _vp, _vp]
         L.cpfft_fill9.argtypes = [_vp, _vp, _dp]
         L.cpfft_mean9.argtypes = [_vp, _vp, _dp]
+        L.cpfft_f2disp.argtypes = [_vp, _vp, _dp, _dp]
         L.cpfft_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong)]
         L.cpfft_kernel_times.argtypes = [_vp, _dp]
         L.cpfft_set_timing.argtypes = [_vp, ctypes.c_int]
@@ -252,6 +253,17 @@ class Solver:
         out = np.zeros(9)
         _check(lib().cpfft_mean9(self.h, self.field(x), _p(out)), "cpfft_mean9")
         return out
+
+    def f2disp(self, f=None, lengths=(1.0, 1.0, 1.0)):
+        """Nodal displacements [(N+1)^3][3] of the brick mesh from the
+        deformation gradient in field f (default FN1), as f2disp.f."""
+        P = self.N + 1
+        u = np.empty(3 * P ** 3)
+        ln = np.ascontiguousarray(lengths, dtype=np.float64)
+        assert ln.size == 3
+        _check(lib().cpfft_f2disp(self.h, self.field(FN1 if f is None else f), _p(ln), _p(u)),
+               "cpfft_f2disp")
+        return u.reshape(P ** 3, 3)
 
     def sync(self):
         _check(lib().cpfft_sync(self.h), "cpfft_sync")

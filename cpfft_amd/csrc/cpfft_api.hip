@@ -68,6 +68,7 @@ struct cpfft_ctx {
   double* hdc = nullptr;
   double* hm = nullptr;
   double* hpq = nullptr;  // p_i . (K4:p_i) block sums of the K4 pass [9][blocks]
+  double* f2d_cos = nullptr;  // cosine table of cpfft_f2disp, [(N+1)^2]
   CGScalars* hcg9 = nullptr;
   // learned warm start
   MlpBuffers mlp;
@@ -138,6 +139,7 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (c->hdc) (void)hipFree(c->hdc);
   if (c->hm) (void)hipFree(c->hm);
   if (c->hpq) (void)hipFree(c->hpq);
+  if (c->f2d_cos) (void)hipFree(c->f2d_cos);
   {
     MlpBuffers& b = c->mlp;
     void* mb[] = {b.params, b.pbf, b.grads, b.m, b.v, b.norm, b.logX, b.logT, b.Xb, b.Xf, b.Tb,
@@ -458,6 +460,33 @@ extern "C" int cpfft_mean9(cpfft_ctx* c, const double* x, double out[9]) {
   int rc = read_res(c, 9);
   if (rc) return rc;
   for (int i = 0; i < 9; ++i) out[i] = c->hres[i];
+  return 0;
+}
+
+// f2disp (f2disp.f:16-175): the spectrum buffers (2 * 9 * S1 complex) hold
+// the right-hand side, the transforms and u before the copy to the host
+extern "C" int cpfft_f2disp(cpfft_ctx* c, const double* F, const double lengths[3], double* u) {
+  if (!c || check_ptr(F)) return -1;
+  if (!lengths || !u) return fail(-1, "cpfft_f2disp: null lengths or output");
+  for (int a = 0; a < 3; ++a)
+    if (!(lengths[a] > 0.0)) return fail(-1, "cpfft_f2disp: box lengths must be positive");
+  const int N = c->N;
+  const long P = N + 1, P3 = P * P * P;
+  const size_t have = 2 * 9 * (size_t)c->geom.S1 * 16;
+  if (N < 2 || f2disp_scratch_doubles(N) * sizeof(double) > have)
+    return fail(-1, "cpfft_f2disp: grid too small");
+  if (!c->f2d_cos) {
+    HIP_OK(hipMalloc(&c->f2d_cos, P * P * sizeof(double)));
+    launch_f2disp_cos(N, c->f2d_cos, c->stream);
+  }
+  double* work = reinterpret_cast<double*>(c->spec);
+  // the last transform leaves x in the first half of the scratch, so u
+  // goes to the second
+  double* ud = work + 3 * P3;
+  launch_f2disp(N, F, lengths, c->f2d_cos, work, ud, c->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(u, ud, 3 * P3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

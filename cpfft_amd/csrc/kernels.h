@@ -33,6 +33,14 @@ struct SpecGeom {
   PencilTune tune;  // launch shapes of the planned path
 };
 
+// Displacement reconstruction (f2disp.hip): the cosine table C[(N+1)^2]
+// once, then u (device, [3 (N+1)^3] node-major) from F; work holds
+// f2disp_scratch_doubles(N) doubles and must not overlap u.
+size_t f2disp_scratch_doubles(int N);
+void launch_f2disp_cos(int N, double* C, hipStream_t s);
+void launch_f2disp(int N, const double* F, const double len[3], const double* C, double* work,
+                   double* u, hipStream_t s);
+
 // Gamma:(K4:F) or Gamma:F (K4 == nullptr).  spec: 2*9*S1 complex scratch.
 // tw: the N twiddles exp(-2 pi i k/N).  Uses the compile-time-planned path
 // of fft_pencil.hip where it exists for N, else the generic runtime plan.

@@ -111,6 +111,13 @@ module cpfft_bindings
        integer(c_int) :: rc
      end function cpfft_mean9
 
+     function cpfft_f2disp(ctx, F, lengths, u) bind(C, name="cpfft_f2disp") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx, F
+       real(c_double) :: lengths(3), u(*)
+       integer(c_int) :: rc
+     end function cpfft_f2disp
+
      function cpfft_counters(ctx, out) bind(C, name="cpfft_counters") result(rc)
        import :: c_ptr, c_int, c_long_long
        type(c_ptr), value :: ctx

@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--homo", action="store_true", help="also time one tangent_homo call")
     ap.add_argument("--mlp", action="store_true", help="also time the warm-start network")
     ap.add_argument("--tune", default="", help="FFT launch shapes a,a0,b,c (default: untuned)")
+    ap.add_argument("--f2disp", action="store_true",
+                    help="also time the displacement reconstruction (host copy included)")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     V = wl.N ** 3
@@ -119,6 +121,14 @@ def main():
         s.sync()
         res["mlp_grad_ms_wall"] = (time.perf_counter() - t0) * 1e3 / args.reps
         res["mlp_rows"] = rows
+    if args.f2disp:
+        s.upload(FN1, np.eye(3).reshape(9, 1) + 1e-3 * rng.standard_normal((9, V)))
+        s.f2disp(FN1)  # first call builds the cosine table
+        t0 = time.perf_counter()
+        for _ in range(args.reps):
+            s.f2disp(FN1)
+        res["f2disp_ms_wall"] = (time.perf_counter() - t0) * 1e3 / args.reps
+        res["f2disp_nodes"] = (wl.N + 1) ** 3
     if not args.skip_material:
         s.upload(FN1, np.tile(np.array([1.0005, 0, 0, 0, 0.99985, 0, 0, 0, 0.99985])[:, None],
                               (1, V)))

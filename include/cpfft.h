@@ -145,6 +145,18 @@ int cpfft_fill9(cpfft_ctx* ctx, double* x, const double v[9]);
 /* P_bar(ii) = sum_jj Pn1(jj,ii) / N3 (FFT_nr3.f:127-134) */
 int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
 
+/* ---- displacement output (f2disp.f:16-175) -----------------------------
+ * Nodal displacements of the (N+1)^3-node brick mesh from the deformation
+ * gradient F (device, [9][V], component 3i+j = F_ij), box edge lengths
+ * lengths[3] (the deck's sizes l_x, l_y, l_z): the reference's system
+ * A x = b (form_BTB 2x2x2 Gauss, one-point right-hand side, node 1 held at
+ * zero), solved in closed form by discrete cosine transforms instead of
+ * PARDISO; u = x - X.  u (host) receives 3 (N+1)^3 doubles, node-major
+ * (u[3 n + i], n = ix + (N+1) iy + (N+1)^2 iz), as the reference's u
+ * array.  Uses the context's spectrum buffers as scratch.  -1 on a bad
+ * argument.                                                                */
+int cpfft_f2disp(cpfft_ctx* ctx, const double* F, const double lengths[3], double* u);
+
 /* ---- execution options (no counterpart in the reference) -------------- */
 /* on != 0: cpfft_pcg and cpfft_tangent_homo replay their chunks of eight CG
  * iterations from HIP graphs captured on first use (one per solution
