@@ -68,7 +68,7 @@ struct cpfft_ctx {
   double* hdc = nullptr;
   double* hm = nullptr;
   double* hpq = nullptr;  // p_i . (K4:p_i) block sums of the K4 pass [9][blocks]
-  double* f2d_cos = nullptr;  // cosine table of cpfft_f2disp, [(N+1)^2]
+  double* f2d_cos = nullptr;  // cosine table of cpfft_f2disp
   CGScalars* hcg9 = nullptr;
   // learned warm start
   MlpBuffers mlp;
@@ -476,7 +476,7 @@ extern "C" int cpfft_f2disp(cpfft_ctx* c, const double* F, const double lengths[
   if (N < 2 || f2disp_scratch_doubles(N) * sizeof(double) > have)
     return fail(-1, "cpfft_f2disp: grid too small");
   if (!c->f2d_cos) {
-    HIP_OK(hipMalloc(&c->f2d_cos, P * P * sizeof(double)));
+    HIP_OK(hipMalloc(&c->f2d_cos, f2disp_cos_doubles(N) * sizeof(double)));
     launch_f2disp_cos(N, c->f2d_cos, c->stream);
   }
   double* work = reinterpret_cast<double*>(c->spec);

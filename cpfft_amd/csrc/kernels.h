@@ -33,10 +33,12 @@ struct SpecGeom {
   PencilTune tune;  // launch shapes of the planned path
 };
 
-// Displacement reconstruction (f2disp.hip): the cosine table C[(N+1)^2]
-// once, then u (device, [3 (N+1)^3] node-major) from F; work holds
-// f2disp_scratch_doubles(N) doubles and must not overlap u.
+// Displacement reconstruction (f2disp.hip): the cosine table (of
+// f2disp_cos_doubles(N) doubles) once, then u (device, [3 (N+1)^3]
+// node-major) from F; work holds f2disp_scratch_doubles(N) doubles and must
+// not overlap u.
 size_t f2disp_scratch_doubles(int N);
+size_t f2disp_cos_doubles(int N);
 void launch_f2disp_cos(int N, double* C, hipStream_t s);
 void launch_f2disp(int N, const double* F, const double len[3], const double* C, double* work,
                    double* u, hipStream_t s);
