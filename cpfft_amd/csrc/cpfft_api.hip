@@ -248,7 +248,8 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   // p.q partial sums: pass D's (gkdf_dot_parts) or pass A's (N*N/2, one per
   // pencil pair)
   HIP_OK(hipMalloc(&c->dpart, std::max({1, gkdf_dot_parts(N), N * N}) * sizeof(double)));
-  HIP_OK(hipMalloc(&c->rpart, std::max(1, gkdf_dot_parts(N)) * sizeof(double)));
+  // r.r partial sums of up to nine systems (tangent_homo's right-hand sides)
+  HIP_OK(hipMalloc(&c->rpart, 9 * (size_t)std::max(1, gkdf_dot_parts(N)) * sizeof(double)));
   HIP_OK(hipMalloc(&c->dres, 16 * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hres, 16 * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcg, sizeof(CGScalars)));
@@ -839,6 +840,8 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
   for (int j = 0; j < kChunk; ++j) {
     launch_k4_contract_multi(V, K4, pp, c->dcg9, 9, c->stream, fused ? &rr : nullptr,
                              apq ? c->hpq : nullptr);
+    // the nine alphas in one launch (one block per right-hand side)
+    if (apq) launch_cg_alpha(c->hpq, nb, c->dcg9, c->stream, 9, nb);
     for (int i = 0; i < 9; ++i) {
       SpecGeom g = c->geom;
       g.skip = &c->dcg9[i].done;
@@ -850,11 +853,9 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
         double* const m = c->hm + 9 * i;
         const double sc = 1.0 / (double)V;
         if (apq) {
-          launch_cg_alpha(c->hpq + (long)i * nb, nb, c->dcg9 + i, c->stream);
           const CGFuse fz{nullptr, nullptr, c->dcg9 + i, nullptr, nullptr, dc, nullptr, 0,
-                          R[i], nullptr, c->rpart};
+                          R[i], nullptr, c->rpart + (long)i * ndp};
           launch_gkdf_pencil(g, Q[i], nullptr, Q[i], c->spec, c->tw, c->stream, &fz);
-          launch_cg_check(c->rpart, ndp, c->dcg9 + i, c->stream, dc, m, sc);
         } else {
           const CGFuse fz{nullptr, nullptr, c->dcg9 + i, Pd[i], c->dpart, dc, nullptr, 0,
                           nullptr, nullptr, nullptr};
@@ -869,6 +870,9 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
         launch_cg_dir_dev(n, c->dcg9 + i, R[i], Pd[i], c->stream);
       }
     }
+    // the nine convergence tests in one launch, after all nine operators
+    if (apq)
+      launch_cg_check(c->rpart, ndp, c->dcg9, c->stream, c->hdc, c->hm, 1.0 / (double)V, 9, ndp);
   }
 }
 

@@ -471,8 +471,11 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_s(const double* __restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 // alpha = rr / (p.q) from the partial sums (same order as k_sum_parts)
+// block b serves system b: partial sums part + b pstride, scalars cs + b
 __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ part, int np,
-                                                     CGScalars* __restrict__ cs) {
+                                                     CGScalars* __restrict__ cs, long pstride) {
+  cs += blockIdx.x;
+  part += blockIdx.x * pstride;
   if (cs->done) return;
   __shared__ double sh[kBlock];
   const double pq = ordered_sum<kBlock>(part, np, sh);
@@ -525,10 +528,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_s(long n, const CGScalars*
 }
 // |r| test from the r.r partial sums (FFT_nr3.f:304-306); with m, first
 // m[c] += alpha * dc[c] * scale (this iteration's alpha)
+// block b serves system b (part + b pstride, cs + b, dc and m + 9 b)
 __global__ __launch_bounds__(kBlock) void k_cg_check(const double* __restrict__ part, int np,
                                                      CGScalars* __restrict__ cs,
                                                      const double* __restrict__ dc,
-                                                     double* __restrict__ m, double scale) {
+                                                     double* __restrict__ m, double scale,
+                                                     long pstride) {
+  cs += blockIdx.x;
+  part += blockIdx.x * pstride;
+  if (dc) dc += 9 * blockIdx.x;
+  if (m) m += 9 * blockIdx.x;
   if (cs->done) return;
   __shared__ double sh[kBlock];
   const double rr1 = ordered_sum<kBlock>(part, np, sh);
@@ -553,22 +562,23 @@ void launch_cg_update_check(long n, CGScalars* cs, const double* p, const double
     k_cg_update_s<true><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
   else
     k_cg_update_s<false><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
-  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, dc, m, scale);
+  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, dc, m, scale, 0);
 }
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s) {
-  k_cg_alpha<<<1, kBlock, 0, s>>>(part, np, cs);
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s, int nsys,
+                     long pstride) {
+  k_cg_alpha<<<nsys, kBlock, 0, s>>>(part, np, cs, pstride);
 }
 void launch_cg_check(const double* part, int np, CGScalars* cs, hipStream_t s, const double* dc,
-                     double* m, double scale) {
-  k_cg_check<<<1, kBlock, 0, s>>>(part, np, cs, dc, m, scale);
+                     double* m, double scale, int nsys, long pstride) {
+  k_cg_check<<<nsys, kBlock, 0, s>>>(part, np, cs, dc, m, scale, pstride);
 }
 
 void launch_cg_tail(long n, CGScalars* cs, const double* p, const double* q, double* x,
                     double* r, double* partials, hipStream_t s) {
   k_dot_part_s<<<kRedBlocks, kBlock, 0, s>>>(p, q, n, partials, cs);
-  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs);
+  k_cg_alpha<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, 0);
   k_cg_update_s<true><<<kRedBlocks, kBlock, 0, s>>>(n, cs, p, q, x, r, partials);
-  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, nullptr, nullptr, 0.0);
+  k_cg_check<<<1, kBlock, 0, s>>>(partials, kRedBlocks, cs, nullptr, nullptr, 0.0, 0);
 }
 void launch_cg_dir_dev(long n, const CGScalars* cs, const double* r, double* p, hipStream_t s) {
   k_cg_dir_s<<<grid_for(n), kBlock, 0, s>>>(n, cs, r, p);

@@ -94,13 +94,17 @@ int pencil_candidate(int N, int pass, int variant);
 bool launch_pencil_pass(const SpecGeom& g, int pass, int variant, const double* F,
                         const double* K4, cplx* spec, const cplx* tw, hipStream_t s);
 
-// alpha = rr / sum(part[0..np)) (p.q partial sums)
-void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s);
+// alpha = rr / sum(part[0..np)) (p.q partial sums); for nsys systems at
+// once, system b reading part + b pstride and cs[b] (one block each)
+void launch_cg_alpha(const double* part, int np, CGScalars* cs, hipStream_t s, int nsys = 1,
+                     long pstride = 0);
 // convergence test and beta from the r.r partial sums part[0..np); with
 // m != nullptr also m[c] += alpha * dc[c] * scale for c < 9 (the running sum
-// over iterations of alpha times the mean of K4:p)
+// over iterations of alpha times the mean of K4:p); nsys systems as for
+// launch_cg_alpha, with dc and m advanced by 9 per system
 void launch_cg_check(const double* part, int np, CGScalars* cs, hipStream_t s,
-                     const double* dc = nullptr, double* m = nullptr, double scale = 0.0);
+                     const double* dc = nullptr, double* m = nullptr, double scale = 0.0,
+                     int nsys = 1, long pstride = 0);
 // x += alpha p (xupd; without it x and p are not touched); r -= alpha q
 // with the r.r partial sums, then the convergence test and beta (no p.q, no
 // direction update)
