@@ -21,10 +21,10 @@
 // strided butterfly writes fall on distinct LDS banks.
 #include "kernels.h"
 
-// Products and sums are rounded separately in this file (no contraction
-// into FMAs): the compiler otherwise fuses differently in different
-// instantiations of the same transform (e.g. pass B with 2 or 8 lines per
-// block), and every launch shape must give the same bits.
+// No implicit contraction into FMAs in this file: the compiler fused
+// differently in different instantiations of the same transform (e.g. pass B
+// with 2 or 8 lines per block), and every launch shape must give the same
+// bits.  The transforms' FMAs are written out (cmul, butterfly).
 #pragma clang fp contract(off)
 
 namespace cpfft {
@@ -120,8 +120,10 @@ static_assert(kPencilCands >= 5, "candidate slots");
 
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+// explicit FMAs: fewer instructions than separate products and sums, and
+// the same in every instantiation
 __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
-  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return cplx{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
 }
 // multiply by -i (forward) or +i (inverse)
 template <bool FWD>
@@ -161,20 +163,19 @@ __device__ __forceinline__ void butterfly(cplx* x, const cplx* tw) {
     cplx o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
     dft4<FWD>(e0, e1, e2, e3);
     dft4<FWD>(o0, o1, o2, o3);
+    // X_1 = E_1 + h s1, X_3 = E_3 + h s3 with the sums s below, as FMAs
     const double h = 0.70710678118654752440;  // sqrt(2)/2
-    cplx w1 = FWD ? cplx{h * (o1.re + o1.im), h * (o1.im - o1.re)}
-                  : cplx{h * (o1.re - o1.im), h * (o1.re + o1.im)};
-    cplx w2 = mul_mi<FWD>(o2);
-    cplx w3 = FWD ? cplx{h * (o3.im - o3.re), -h * (o3.re + o3.im)}
-                  : cplx{-h * (o3.re + o3.im), h * (o3.re - o3.im)};
+    const cplx s1 = FWD ? cplx{o1.re + o1.im, o1.im - o1.re} : cplx{o1.re - o1.im, o1.re + o1.im};
+    const cplx s3 = FWD ? cplx{o3.im - o3.re, -(o3.re + o3.im)} : cplx{-(o3.re + o3.im), o3.re - o3.im};
+    const cplx w2 = mul_mi<FWD>(o2);
     x[0] = cadd(e0, o0);
     x[4] = csub(e0, o0);
-    x[1] = cadd(e1, w1);
-    x[5] = csub(e1, w1);
+    x[1] = cplx{fma(h, s1.re, e1.re), fma(h, s1.im, e1.im)};
+    x[5] = cplx{fma(-h, s1.re, e1.re), fma(-h, s1.im, e1.im)};
     x[2] = cadd(e2, w2);
     x[6] = csub(e2, w2);
-    x[3] = cadd(e3, w3);
-    x[7] = csub(e3, w3);
+    x[3] = cplx{fma(h, s3.re, e3.re), fma(h, s3.im, e3.im)};
+    x[7] = cplx{fma(-h, s3.re, e3.re), fma(-h, s3.im, e3.im)};
   } else {
     // direct DFT with the R-th roots of unity from the N-th root table
     cplx y[R];
