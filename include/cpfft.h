@@ -166,7 +166,7 @@ int cpfft_f2disp(cpfft_ctx* ctx, const double* F, const double lengths[3], doubl
 int cpfft_set_graphs(cpfft_ctx* ctx, int on);
 
 /* Autotuner of the compile-time-planned FFT path (N in 16, 32, 48, 64, 96,
- * 128, 256): times up to three launch shapes of each of pass A with K4,
+ * 128, 256): times the candidate launch shapes of each of pass A with K4,
  * pass A without K4, the y passes and pass C on this context's buffers
  * (reps launches each) and keeps the fastest.  The choice is cached per
  * device and N for every context of the process (force != 0 measures
@@ -176,8 +176,10 @@ int cpfft_set_graphs(cpfft_ctx* ctx, int on);
  * tune) for other N.                                                      */
 int cpfft_autotune(cpfft_ctx* ctx, int reps, int force);
 /* shapes in use: out[0] pencils per block of pass A with K4, out[1]
- * without K4, out[2] lines per block of the y passes, out[3] y lines per
- * block of pass C (0 where N has no planned path)                         */
+ * without K4, out[2] lines per block of the y passes, out[3] component
+ * lines per block of pass C: 24, 12 or 6 (one tensor row of 8, 4 or 2 y
+ * lines, transforms in registers) for N = 64, 128, 256, else 18 or 9 (all
+ * nine components of 2 or 1 y lines); 0 where N has no planned path      */
 int cpfft_get_tune(cpfft_ctx* ctx, int out[4]);
 /* set them (each must be one of the candidates; -1 otherwise)             */
 int cpfft_set_tune(cpfft_ctx* ctx, const int in[4]);
