@@ -116,6 +116,17 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
     }
 
 
+def workload_text(wl):
+    """Crystal and loading of a workload, e.g. 'FCC cubic Voce n=20, uniaxial
+    tension 1e-3/s'."""
+    crystal = "FCC" if wl.material.slip_set == 1 else "BCC24"
+    span = wl.strain * (4 * wl.cycles if wl.cycles > 0 else 1)  # strain travelled
+    rate = f"{span / (wl.nstep * wl.dt):.0e}".replace("e-0", "e-")
+    load = (f"uniaxial tension-compression +-{100 * wl.strain:g}% x{wl.cycles}, {rate}/s"
+            if wl.cycles > 0 else f"uniaxial tension {rate}/s")
+    return f"{crystal} cubic Voce n=20, {load}"
+
+
 def timed_steps(s, wl, BC_all, isNBC, W, K):
     """Time load steps W+1..W+K on a solver that has run steps 1..W;
     returns (wall, rc, Pbar, cnt, times, counter deltas, kernel avg ms, counts)."""
@@ -303,7 +314,7 @@ def main():
     gk_ms = avg[0]
     achieved = CG_ITERATION_BYTES_PER_VOXEL * V / (gk_ms * 1e-3) / 1e9 if gk_ms > 0 else 0.0
     out = {
-        "metric": "converged load steps/s at 128^3 (plain method; warm start in warm_start)",
+        "metric": f"converged load steps/s at {wl.N}^3 (plain method; warm start in warm_start)",
         "value": steps_per_s,
         "unit": "load steps/s",
         "n_gpus": 1,
@@ -316,8 +327,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"{wl.name}: {wl.N}^3 periodic Voronoi, {wl.ngrains} grains, FCC cubic "
-                         f"Voce n=20, uniaxial tension 1e-3/s, dt={wl.dt}s, steps {W + 1}-{nstep}"),
+            "workload": (f"{wl.name}: {wl.N}^3 periodic Voronoi, {wl.ngrains} grains, "
+                         f"{workload_text(wl)}, dt={wl.dt}s, steps {W + 1}-{nstep}"),
             "grid": wl.N, "grains": wl.ngrains, "gamma_mode": wl.gamma_mode,
             "jac_exact": wl.material.jac_exact, "tolPCG": wl.tolPCG, "tolNR": wl.tolNR,
             "fft_launch_shapes": shapes,
