@@ -102,7 +102,7 @@ constexpr int xl_cand(int n, int i) {
   int k = 0;
   if (regc_ok(n)) {
     for (int yb = 8; yb >= 2; yb /= 2)
-      if (lds_fits(3 * yb, n)) {
+      if ((3 * yb * lstride(n) + n) * 16 + 2 * yb * n * 8 <= 65536) {  // lines, twiddles, weights
         if (k == i) return 3 * yb;
         ++k;
       }
@@ -526,6 +526,7 @@ __global__ __launch_bounds__(24 * YB) void kC_xr(cplx* __restrict__ T2,
   static_assert(regc_ok(N), "register pass C needs N = 8 M, M a multiple of 8");
   __shared__ cplx tw[N];
   __shared__ cplx buf[3 * YB * LS];
+  __shared__ double sw[2][YB * N];  // projection weights at (t, x), shared by the row's lines
   const int y0 = blockIdx.x * YB, kz = blockIdx.y, row = blockIdx.z;
   const int L = threadIdx.x / R1, g = threadIdx.x % R1;  // line L = jc * YB + t
   const int jc = L / YB, t = L - jc * YB, c = 3 * row + jc;
@@ -537,6 +538,11 @@ __global__ __launch_bounds__(24 * YB) void kC_xr(cplx* __restrict__ T2,
   for (int gc = 0; gc < G; ++gc)
 #pragma unroll
     for (int u = 0; u < R1; ++u) x[gc][u] = gl[g + R1 * gc + M * u];
+  // the weights, while the loads are in flight (read after the barriers)
+  for (int i = threadIdx.x; i < YB * N; i += T) {
+    const int tt = i / N, xx = i - tt * N;
+    proj_weights(N, gamma_mode, xx, y0 + tt, kz, sw[0][i], sw[1][i]);
+  }
   // forward: first pass on columns g + 8 gc, exchange, later passes
 #pragma unroll
   for (int gc = 0; gc < G; ++gc) {
@@ -562,7 +568,8 @@ __global__ __launch_bounds__(24 * YB) void kC_xr(cplx* __restrict__ T2,
     for (int u = 0; u < R1; ++u) {
       const int xx = g + R1 * gc + M * u;
       const cplx A3[3] = {r0[pidx(xx)], r0[YB * LS + pidx(xx)], r0[2 * YB * LS + pidx(xx)]};
-      x[gc][u] = project1(N, gamma_mode, xx, y0 + t, kz, A3, jc);
+      x[gc][u] = project1w(N, gamma_mode, xx, y0 + t, kz, A3, jc, sw[0][t * N + xx],
+                           sw[1][t * N + xx]);
     }
 #pragma unroll
   for (int gc = 0; gc < G; ++gc) butterfly<R1, false, N>(x[gc], tw);

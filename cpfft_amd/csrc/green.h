@@ -53,14 +53,6 @@ __device__ inline void gamma_acc(const double q[3], const cplx A[9], cplx out[9]
     for (int j = 0; j < 3; ++j) gamma_term(aq, q[j], s, out[3 * i + j]);
   }
 }
-// component (i, j) of the same sum, from row i of A only
-__device__ inline void gamma_acc1(const double q[3], const cplx A3[3], int j, cplx& o, double w) {
-  double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-  if (qq <= 1e-10) return;
-  double s = w / qq;
-  gamma_term(gamma_aq(q, A3), j == 0 ? q[0] : j == 1 ? q[1] : q[2], s, o);
-}
-
 __device__ inline int wrapf(int v, int N) {
   // representative in [-N/2, N/2) for even N, (-N/2, N/2] for odd N
   int h = N / 2;
@@ -94,31 +86,46 @@ __device__ inline void project9(int N, int gamma_mode, int x, int y, int kz, con
   }
 }
 
-// component (i, j) of project9's result, from row i of A (A3 = A[3i..3i+2]):
-// the same operations on the same operands, so the same bits
-__device__ inline cplx project1(int N, int gamma_mode, int x, int y, int kz, const cplx A3[3],
-                                int j) {
-  cplx o{0.0, 0.0};
+// project9 one component at a time, for kernels that hold one tensor row.
+// proj_vectors: the wave vectors of project9 at one frequency (q2 for the
+// reference's even-N operator only); proj_weights: their weights
+// s = w / |q|^2, 0 where project9 adds nothing (|q|^2 <= 1e-10, the
+// textbook operator's Nyquist planes); project1w: component (i, j) of
+// project9's result from row i of A (A3) and those weights.  Where a term is
+// added the operations and operands are project9's; a zero weight adds a
+// signed zero, which leaves the sum's bits as project9 leaves them.
+__device__ inline bool proj_vectors(int N, int gamma_mode, int x, int y, int kz, double q1[3],
+                                    double q2[3]) {
   const bool even = (N % 2) == 0;
   const int f[3] = {x <= N / 2 ? x : x - N, y <= N / 2 ? y : y - N, kz};
-  if (gamma_mode == 0 && even) {
-    double q1[3], q2[3];
+  const bool two = gamma_mode == 0 && even;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      q1[a] = (double)wrapf(f[a] - 1, N);
-      q2[a] = (double)wrapf(-f[a] - 1, N);
-    }
-    gamma_acc1(q1, A3, j, o, 0.5);
-    gamma_acc1(q2, A3, j, o, 0.5);
-  } else {
-    const bool nyq = even && (abs(f[0]) == N / 2 || abs(f[1]) == N / 2 || f[2] == N / 2);
-    if (!nyq) {
-      double q[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) q[a] = (double)wrapf(f[a], N);
-      gamma_acc1(q, A3, j, o, 1.0);
-    }
+  for (int a = 0; a < 3; ++a) {
+    q1[a] = (double)wrapf(two ? f[a] - 1 : f[a], N);
+    q2[a] = (double)wrapf(-f[a] - 1, N);
   }
+  return two;
+}
+__device__ inline void proj_weights(int N, int gamma_mode, int x, int y, int kz, double& s1,
+                                    double& s2) {
+  double q1[3], q2[3];
+  const bool two = proj_vectors(N, gamma_mode, x, y, kz, q1, q2);
+  const bool even = (N % 2) == 0;
+  const int f[3] = {x <= N / 2 ? x : x - N, y <= N / 2 ? y : y - N, kz};
+  const bool nyq = !two && even && (abs(f[0]) == N / 2 || abs(f[1]) == N / 2 || f[2] == N / 2);
+  const double w = two ? 0.5 : 1.0;
+  const double qq1 = q1[0] * q1[0] + q1[1] * q1[1] + q1[2] * q1[2];
+  const double qq2 = q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2];
+  s1 = (nyq || qq1 <= 1e-10) ? 0.0 : w / qq1;
+  s2 = (!two || qq2 <= 1e-10) ? 0.0 : w / qq2;
+}
+__device__ inline cplx project1w(int N, int gamma_mode, int x, int y, int kz, const cplx A3[3],
+                                 int j, double s1, double s2) {
+  cplx o{0.0, 0.0};
+  double q1[3], q2[3];
+  const bool two = proj_vectors(N, gamma_mode, x, y, kz, q1, q2);
+  gamma_term(gamma_aq(q1, A3), j == 0 ? q1[0] : j == 1 ? q1[1] : q1[2], s1, o);
+  if (two) gamma_term(gamma_aq(q2, A3), j == 0 ? q2[0] : j == 1 ? q2[1] : q2[2], s2, o);
   return o;
 }
 
