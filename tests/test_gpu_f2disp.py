@@ -71,3 +71,25 @@ def test_f2disp_rejects_bad_lengths():
     with pytest.raises(C.CpfftError):
         s.f2disp(C.FN1, (1.0, 0.0, 1.0))
     s.close()
+
+
+def test_f2disp_after_load_steps_against_oracle():
+    """The deformation gradient the solver leaves in FN1 after real load
+    steps (16^3, 8 grains, mixed-BC tension), reconstructed on the GPU and
+    by the oracle from the same downloaded field."""
+    from cpfft_amd.configs import WORKLOADS
+    wl = WORKLOADS["c1_16"]
+    BC, isNBC = wl.bc()
+    s = C.Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    rc, Pbar, _, _ = s.compute(BC[:20], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    assert rc == 0
+    F = s.download(C.FN1)
+    L = (1.0, 1.0, 1.0)
+    u = s.f2disp(C.FN1, L)
+    ref = O.f2disp(F, wl.N, L)
+    err = np.abs(u - ref).max()
+    print(f"\n[f2disp] c1_16 after 20 steps: max|u|={np.abs(ref).max():.3e} max err={err:.3e}",
+          flush=True)
+    assert np.abs(ref).max() > 1e-3  # a real deformation
+    assert err <= 1e-10 * max(L)
+    s.close()
