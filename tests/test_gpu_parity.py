@@ -224,12 +224,13 @@ def _gradient_field(N, seed):
     return F
 
 
-@pytest.mark.parametrize("N", [96, 128])
+@pytest.mark.parametrize("N", [96, 128, 256])
 def test_gkdf_projection_properties_full_size(N):
     """At the benchmark grid sizes, with no oracle: the textbook Gamma is a
     projection (Gamma:Gamma:F = Gamma:F), it returns a compatible
     (gradient) field unchanged, it removes the mean, and K4 = identity
-    reproduces Gamma:F."""
+    reproduces Gamma:F.  256^3 is the 256^3 workload's grid (about 20 GB of
+    host arrays)."""
     V = N ** 3
     s, _ = solver(N, 1, ngr=8)
     rng = np.random.default_rng(N)
@@ -252,4 +253,32 @@ def test_gkdf_projection_properties_full_size(N):
     s.upload(C.DFM, Fg)
     s.gkdf(C.DFM, C.B, 0)
     assert rel(s.download(C.B), Fg) < 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_cg_residual_full_size(N):
+    """The fused CG at the benchmark grids (alpha from pass A's p.(K4:p)
+    sums, x and r updated in pass D), checked without an oracle: the
+    returned x satisfies Gamma:(K4:x) = b to the requested tolerance when
+    the residual is formed again by a separate operator application, and
+    the reported relative residual is that one.  K4 is the elastic tangent
+    of a 16-grain polycrystal."""
+    s, _ = solver(N, 1, ngr=16)
+    s.drive_eps_sig(1, 0)
+    rng = np.random.default_rng(5 * N)
+    s.upload(C.DFM, 1e-3 * rng.standard_normal((9, N ** 3)))
+    s.gkdf(C.DFM, C.B, 1)
+    s.dscal(-1.0, C.B)
+    b = s.download(C.B)
+    tol = 1e-8
+    rc, it, relres = s.pcg(C.B, C.WORK, tol)
+    print(f"\n[cg] N={N}: {it} iterations, reported relative residual {relres:.3e}", flush=True)
+    assert rc == 0 and it > 5
+    s.gkdf(C.WORK, C.DFM, 1)
+    r = b - s.download(C.DFM)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"[cg] N={N}: recomputed relative residual {true_rel:.3e}", flush=True)
+    assert true_rel <= tol
+    assert abs(true_rel - relres) <= 1e-3 * tol
     s.close()
