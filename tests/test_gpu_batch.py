@@ -135,7 +135,7 @@ def test_batch_continues_runs():
         s.close()
 
 
-@pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128])
+@pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128, 256])
 def test_every_tuned_shape_gives_identical_results(N):
     """The autotuner's candidates (cpfft_set_tune accepts exactly those)
     change only which block transforms which lines, and for N = 64 and 128
