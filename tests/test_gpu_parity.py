@@ -86,16 +86,18 @@ def _fields_near_identity(rng, V, amp, uniaxial=0.0):
     return F
 
 
-@pytest.mark.parametrize("jac_exact", [0, 1])
-def test_drive_eps_sig(jac_exact):
+@pytest.mark.parametrize("slip_set,jac_exact", [(1, 0), (1, 1), (8, 1)])
+def test_drive_eps_sig(slip_set, jac_exact):
     """Two load steps of per-voxel updates on 8^3 voxels: elastic predictor,
-    plastic Newton updates, a second step from the first step's state."""
+    plastic Newton updates, a second step from the first step's state.
+    slip_set 1: FCC (12 systems); 8: BCC24 (24 systems, the 256^3
+    workload's crystal)."""
     N = 8
     V = N ** 3
-    rng = np.random.default_rng(2 + jac_exact)
-    mat = C.Material(jac_exact=jac_exact)
+    rng = np.random.default_rng(2 + jac_exact + slip_set)
+    mat = C.Material(slip_set=slip_set, jac_exact=jac_exact)
     s, ang = solver(N, 1, ngr=8, mat=mat)
-    oc = O.crystal(jac_exact=jac_exact)
+    oc = O.crystal(slip_set=slip_set, jac_exact=jac_exact)
     Fn = _fields_near_identity(rng, V, 0.0)
     Fn1 = _fields_near_identity(rng, V, 1e-4, uniaxial=4e-4)
     sn = O.state_init(oc, V)
