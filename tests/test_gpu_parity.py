@@ -178,6 +178,29 @@ def test_load_steps(N, mode, nstep, ngr):
     s.close()
 
 
+def test_load_steps_bcc24_cyclic():
+    """The 256^3 workload's crystal (BCC24) and loading type (tension then
+    compression) on an 8^3, 4-grain RVE: one cycle of +-0.1% (past yield) in
+    16 steps of 2.5e-4, the 256^3 workload's increment, through the Fortran
+    driver against the oracle's FFT_nr3."""
+    nstep = 16
+    wl = C.Workload("bcc", 8, 4, 20261202 + 97, nstep, 0.001, cycles=1,
+                    material=C.Material(slip_set=8))
+    ang = wl.angles()
+    BC, isNBC = wl.bc()
+    s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    rc, Pbar, cnt, _ = s.compute(BC, isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    oc = O.crystal(slip_set=8, jac_exact=wl.material.jac_exact)
+    Pbo, cnto, rco, _ = O.fft_nr3(oc, wl.N, ang, wl.dt, BC, isNBC, wl.tolPCG, wl.tolNR,
+                                   wl.maxIter, gamma_mode=wl.gamma_mode)
+    print(f"\n[bcc24] Pbar_xx per step {np.round(Pbar[:, 0], 3).tolist()}", flush=True)
+    assert rc == 0 and rco == 0
+    assert Pbar[:, 0].max() > 0 > Pbar[:, 0].min()  # tension and compression
+    assert np.array_equal(cnt[:, [0, 2, 3]], cnto[:, [0, 2, 3]])
+    assert rel(Pbar, Pbo) < 1e-9
+    s.close()
+
+
 def test_full_run_c1_16_against_oracle_fixture():
     """The 16^3, 8-grain workload (configs c1_16) run to its end, 80 load
     steps of mixed-BC tension to 2%, through the Fortran driver, against the
