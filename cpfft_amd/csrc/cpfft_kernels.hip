@@ -599,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
   if (!any) return;  // the same in every thread of the launch
   double k[81];
 #pragma unroll
-  for (int j = 0; j < 81; ++j) k[j] = valid ? K4[j * V + e] : 0.0;
+  for (int j = 0; j < 81; ++j) k[j] = valid ? __builtin_nontemporal_load(K4 + j * V + e) : 0.0;
   // r and p of right-hand side i in buffer i % 2: those of the next running
   // one are loaded before the current one is computed and stored, so their
   // latency overlaps the work (the done flags are the same in every thread)
@@ -608,8 +608,8 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
   auto load = [&](int i, double (&r9)[9], double (&p9)[9]) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      if (DIR) r9[c] = valid ? rr.r[i][c * V + e] : 0.0;
-      p9[c] = valid ? pp.p[i][c * V + e] : 0.0;
+      if (DIR) r9[c] = valid ? __builtin_nontemporal_load(rr.r[i] + c * V + e) : 0.0;
+      p9[c] = valid ? __builtin_nontemporal_load(pp.p[i] + c * V + e) : 0.0;
     }
   };
   // pqpart: per right-hand side, the wave sums of p_i . (K4:p_i)
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
         f[c] = rb[i & 1][c] + beta * pb[i & 1][c];
-        if (valid) pw[c * V + e] = f[c];
+        if (valid) __builtin_nontemporal_store(f[c], pw + c * V + e);
       }
     } else {
 #pragma unroll
@@ -637,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_k4_contract_multi(long V, const doub
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
       const double q = ddot42n_row(k + 9 * r, f);
-      if (valid) pp.q[i][r * V + e] = q;
+      if (valid) __builtin_nontemporal_store(q, pp.q[i] + r * V + e);
       pq += q * f[r];
     }
     if (pqpart) {
