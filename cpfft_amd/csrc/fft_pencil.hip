@@ -295,11 +295,12 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
           if constexpr (DIR) {
-            const double v = R[d * V + e0 + p] + beta * Pd[d * V + e0 + p];
-            Pd[d * V + e0 + p] = v;
+            const double v = __builtin_nontemporal_load(R + d * V + e0 + p) +
+                             beta * __builtin_nontemporal_load(Pd + d * V + e0 + p);
+            __builtin_nontemporal_store(v, Pd + d * V + e0 + p);
             f[u][d] = v;
           } else {
-            f[u][d] = F[d * V + e0 + p];
+            f[u][d] = __builtin_nontemporal_load(F + d * V + e0 + p);
           }
         }
       }
@@ -313,7 +314,8 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
       if (inb(p)) {
 #pragma unroll
         for (int d = 0; d < NK; ++d)
-          k[u][d] = HASK ? K4[(9 * c + d) * V + e0 + p] : F[c * V + e0 + p];
+          k[u][d] = HASK ? __builtin_nontemporal_load(K4 + (9 * c + d) * V + e0 + p)
+                         : __builtin_nontemporal_load(F + c * V + e0 + p);
       }
     }
   };
@@ -628,11 +630,11 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
   for (int j = 0; j < PP; ++j) {
     const int p = threadIdx.x + j * kT;
     const long i = c * V + e0 + p;
-    if constexpr (DOT) pv[j] = p < P ? Pd[i] : 0.0;
+    if constexpr (DOT) pv[j] = p < P ? __builtin_nontemporal_load(Pd + i) : 0.0;
     if constexpr (UPD) {
-      rv[j] = p < P ? R[i] : 0.0;
-      pv[j] = xu && p < P ? Pd[i] : 0.0;
-      xv[j] = xu && p < P ? X[i] : 0.0;
+      rv[j] = p < P ? __builtin_nontemporal_load(R + i) : 0.0;
+      pv[j] = xu && p < P ? __builtin_nontemporal_load(Pd + i) : 0.0;
+      xv[j] = xu && p < P ? __builtin_nontemporal_load(X + i) : 0.0;
     }
   }
   __syncthreads();
@@ -649,12 +651,12 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
       const double q = ((yy & 1) ? v.im : v.re) * sc;
       const long i = c * V + e0 + p;
       if constexpr (UPD) {
-        if (xu) X[i] = xv[j] + a * pv[j];
+        if (xu) __builtin_nontemporal_store(xv[j] + a * pv[j], X + i);
         const double rn = rv[j] - a * q;
-        R[i] = rn;
+        __builtin_nontemporal_store(rn, R + i);
         acc += rn * rn;
       } else {
-        out[i] = q;
+        __builtin_nontemporal_store(q, out + i);
         if constexpr (DOT) acc += q * pv[j];
       }
     }
