@@ -388,6 +388,15 @@ __global__ __launch_bounds__(threads_z(TY * N, HASK)) void kA_zfwd(const double*
   }
 }
 
+// Non-temporal 16-byte load of one complex value: every word of the half
+// spectrum is read once per launch.  Its stores stay ordinary, so that the
+// next pass finds what it reads in the memory-side cache.
+typedef double cplx_nt __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cplx ld_nt(const cplx* a) {
+  const cplx_nt v = __builtin_nontemporal_load(reinterpret_cast<const cplx_nt*>(a));
+  return cplx{v.x, v.y};
+}
+
 // ---- passes B, B': FFT along y with a transpose ---------------------------
 // FWD: T1[c][kz][x][y] -> T2[c][kz][y][x]; inverse: T2 -> T1.
 // grid (N/TX, NZ, 9)
@@ -405,12 +414,12 @@ __global__ __launch_bounds__(kT) void kB_y(const cplx* __restrict__ src, cplx* _
   if constexpr (FWD) {
     for (int i = threadIdx.x; i < TX * N; i += kT) {
       const int t = i / N, y = i - t * N;
-      buf[t * LS + pidx(y)] = src[plane + (long)(x0 + t) * N + y];
+      buf[t * LS + pidx(y)] = ld_nt(src + plane + (long)(x0 + t) * N + y);
     }
   } else {
     for (int i = threadIdx.x; i < TX * N; i += kT) {
       const int y = i / TX, t = i - y * TX;
-      buf[t * LS + pidx(y)] = src[plane + (long)y * N + x0 + t];
+      buf[t * LS + pidx(y)] = ld_nt(src + plane + (long)y * N + x0 + t);
     }
   }
   __syncthreads();
@@ -539,7 +548,7 @@ __global__ __launch_bounds__(24 * YB) void kC_xr(cplx* __restrict__ T2,
 #pragma unroll
   for (int gc = 0; gc < G; ++gc)
 #pragma unroll
-    for (int u = 0; u < R1; ++u) x[gc][u] = gl[g + R1 * gc + M * u];
+    for (int u = 0; u < R1; ++u) x[gc][u] = ld_nt(gl + g + R1 * gc + M * u);
   // the weights, while the loads are in flight (read after the barriers)
   for (int i = threadIdx.x; i < YB * N; i += T) {
     const int tt = i / N, xx = i - tt * N;
@@ -653,7 +662,7 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
       if constexpr (UPD) {
         if (xu) __builtin_nontemporal_store(xv[j] + a * pv[j], X + i);
         const double rn = rv[j] - a * q;
-        __builtin_nontemporal_store(rn, R + i);
+        R[i] = rn;  // an ordinary store: the next pass A reads r soon after
         acc += rn * rn;
       } else {
         __builtin_nontemporal_store(q, out + i);
