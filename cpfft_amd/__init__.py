@@ -43,7 +43,7 @@ EXPORTS = [
     "cpfft_gkdf", "cpfft_pcg",
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
     "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_f2disp", "cpfft_counters",
-    "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_autotune",
+    "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_set_cg_maxit", "cpfft_autotune",
     "cpfft_get_tune", "cpfft_set_tune",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
     "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_stats",
@@ -87,6 +87,7 @@ def lib():
         L.cpfft_field.argtypes = [_vp, ctypes.c_int]
         L.cpfft_field.restype = _vp
         L.cpfft_set_graphs.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_set_cg_maxit.argtypes = [_vp, ctypes.c_int]
         L.cpfft_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.cpfft_get_tune.argtypes = [_vp, _ip]
         L.cpfft_set_tune.argtypes = [_vp, _ip]
@@ -274,8 +275,13 @@ class Solver:
         return list(out)
 
     def set_graphs(self, on):
-        """replay CG iteration chunks from HIP graphs (results unchanged)"""
+        """replay CG iteration chunks from HIP graphs (results unchanged):
+        True or 1 captures the driver's graphs now, 2 at first use"""
         _check(lib().cpfft_set_graphs(self.h, int(on)), "cpfft_set_graphs")
+
+    def set_cg_maxit(self, maxit):
+        """iteration limit of the CG solves (fftPcg's maxIter, 1000)"""
+        _check(lib().cpfft_set_cg_maxit(self.h, int(maxit)), "cpfft_set_cg_maxit")
 
     def autotune(self, reps=10, force=False):
         """choose the fastest launch shapes of the planned FFT path (results

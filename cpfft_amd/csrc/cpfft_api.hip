@@ -82,11 +82,17 @@ struct cpfft_ctx {
   void* orients = nullptr;
   int32_t* oid = nullptr;
   long long* dcnt = nullptr;
+  // pinned host mirrors of device results read back on the context's
+  // stream: counters before [0..8) and after [8..16) a stress update, and
+  // tangent_homo's accumulated means (81)
+  long long* hcnt = nullptr;
+  double* hhm = nullptr;
   long long cg_iters = 0, gkdf_calls = 0, des_calls = 0;
   double drv[99] = {};  // driver state: barF_t, P_bar, C_homo
   // CG iteration chunks replayed from HIP graphs (cpfft_set_graphs): one
   // per solution pointer for cpfft_pcg, one for tangent_homo
   int use_graphs = 0;
+  int cg_maxit = 1000;  // fftPcg's maxIter (FFT_nr3.f:224)
   std::vector<std::pair<const double*, hipGraphExec_t>> pcg_graphs;
   hipGraphExec_t homo_graph = nullptr;
   // kernel timing: event pairs recorded on the stream without host
@@ -132,6 +138,8 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
   if (c->hres) (void)hipHostFree(c->hres);
   if (c->hcg) (void)hipHostFree(c->hcg);
   if (c->hcg9) (void)hipHostFree(c->hcg9);
+  if (c->hcnt) (void)hipHostFree(c->hcnt);
+  if (c->hhm) (void)hipHostFree(c->hhm);
   for (int a = 0; a < 4; ++a)
     for (int i = 0; i < 9; ++i)
       if (c->th[a][i]) (void)hipFree(c->th[a][i]);
@@ -262,7 +270,9 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipMalloc(&c->hpq, 9L * k4_contract_multi_blocks(V) * sizeof(double)));
   HIP_OK(hipHostMalloc(&c->hcg9, 9 * sizeof(CGScalars), hipHostMallocDefault));
   HIP_OK(hipMalloc(&c->dcnt, 8 * sizeof(long long)));
-  HIP_OK(hipMemset(c->dcnt, 0, 8 * sizeof(long long)));
+  HIP_OK(hipMemsetAsync(c->dcnt, 0, 8 * sizeof(long long), c->stream));
+  HIP_OK(hipHostMalloc(&c->hcnt, 16 * sizeof(long long), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc(&c->hhm, 81 * sizeof(double), hipHostMallocDefault));
   // twiddles exp(-2 pi i k/N), computed in long double
   std::vector<cplx> htw(N);
   const long double pil = 3.141592653589793238462643383279502884L;
@@ -271,7 +281,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
     htw[k] = cplx{(double)cosl(a), (double)-sinl(a)};
   }
   HIP_OK(hipMalloc(&c->tw, N * sizeof(cplx)));
-  HIP_OK(hipMemcpy(c->tw, htw.data(), N * sizeof(cplx), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpyAsync(c->tw, htw.data(), N * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
   rc = cpfft_reset(c);
   if (rc) { cpfft_destroy(c); return rc; }
   *out = c;
@@ -312,27 +322,22 @@ static void driver_state_init(cpfft_ctx* c) {
 extern "C" double* cpfft_driver_state(cpfft_ctx* c) { return c ? c->drv : nullptr; }
 
 extern "C" int cpfft_reset(cpfft_ctx* c) {
+  if (!c) return fail(-1, "cpfft_reset: null context");
   driver_state_init(c);
   const long V = c->V;
-  const int ns = c->nslip, W = CPFFT_NWORDS(ns);
-  std::vector<double> h(9 * V, 0.0);
-  for (long e = 0; e < V; ++e) h[e] = h[4 * V + e] = h[8 * V + e] = 1.0;
-  HIP_OK(hipMemcpy(c->fields[CPFFT_FN], h.data(), 9 * V * sizeof(double), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(c->fields[CPFFT_FN1], h.data(), 9 * V * sizeof(double), hipMemcpyHostToDevice));
-  std::fill(h.begin(), h.end(), 0.0);
-  for (int f : {CPFFT_PN, CPFFT_PN1, CPFFT_DFM, CPFFT_B, CPFFT_WORK})
-    HIP_OK(hipMemcpy(c->fields[f], h.data(), 9 * V * sizeof(double), hipMemcpyHostToDevice));
-  std::vector<double> s((long)W * V, 0.0);
-  for (long e = 0; e < V; ++e) {
-    s[(CPFFT_W_RP + 0) * V + e] = 1.0;
-    s[(CPFFT_W_RP + 4) * V + e] = 1.0;
-    s[(CPFFT_W_RP + 8) * V + e] = 1.0;
-    s[CPFFT_W_TT * V + e] = c->mp.tau_y + 1.0e-5;  // mm10_init_voche (mm10_a.f:2042-2055)
-  }
-  HIP_OK(hipMemcpy(c->fields[CPFFT_STATE_N], s.data(), W * V * sizeof(double), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(c->fields[CPFFT_STATE_NP1], s.data(), W * V * sizeof(double), hipMemcpyHostToDevice));
-  std::vector<double> z(81 * V, 0.0);
-  HIP_OK(hipMemcpy(c->fields[CPFFT_K4], z.data(), 81 * V * sizeof(double), hipMemcpyHostToDevice));
+  const int W = CPFFT_NWORDS(c->nslip);
+  // device fills on the context's stream: ordered after the work already
+  // queued there, and no host copy of the fields
+  const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  launch_fill9(V, id, c->fields[CPFFT_FN], c->stream);
+  launch_fill9(V, id, c->fields[CPFFT_FN1], c->stream);
+  for (int f : {CPFFT_PN, CPFFT_PN1, CPFFT_DFM, CPFFT_B, CPFFT_WORK, CPFFT_K4})
+    launch_fill(c->field_len[f], 0.0, c->fields[f], c->stream);
+  // tau_tilde starts at tau_y + 1e-5 (mm10_init_voche, mm10_a.f:2042-2055)
+  for (int f : {CPFFT_STATE_N, CPFFT_STATE_NP1})
+    launch_state_init(V, W, c->mp.tau_y + 1.0e-5, c->fields[f], c->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -547,26 +552,62 @@ static void enqueue_pcg_chunk(cpfft_ctx* c, double* x, int base = 0) {
   c->ev_cur = -1;
 }
 
-// record what enqueue() launches on the context's stream into an
-// executable graph (kernels only: no memset or copy nodes)
-// Capture and instantiation are serialised over the process: contexts
-// driven from several threads (cpfft_compute_batch) then never capture at
-// the same time.
+// Record what enqueue() launches on the context's stream into an
+// executable graph.  Everything the runtime does for a graph happens here,
+// under one process-wide lock, before the graph is first launched:
+// - the stream must not be capturing already;
+// - kernel timing is off during the capture (a recorded event would become
+//   a node, and a full event ring synchronises the stream, which ends the
+//   capture);
+// - capture in thread-local mode; the graph must consist of kernel nodes
+//   only, as many as the first capture of the same chunk (kind, edge,
+//   operator and launch shapes) held, so that a capture that took in a
+//   foreign launch or lost one of its own is an error, not a wrong result;
+// - instantiation, then the upload of the executable graph (its kernel
+//   arguments included) completed by a stream synchronisation.
 static std::mutex g_capture_mu;
+static std::map<std::array<int, 7>, size_t> g_chunk_nodes;
 
 template <class F>
-static int capture_chunk(cpfft_ctx* c, F&& enqueue, hipGraphExec_t* out) {
+static int capture_chunk(cpfft_ctx* c, int kind, F&& enqueue, hipGraphExec_t* out) {
   std::lock_guard<std::mutex> lk(g_capture_mu);
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  HIP_OK(hipStreamIsCapturing(c->stream, &cst));
+  if (cst != hipStreamCaptureStatusNone) return fail(-101, "graph capture: stream already capturing");
+  const int timing = c->timing;
+  c->timing = 0;
   hipGraph_t gr = nullptr;
-  HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue();
+  hipError_t be = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (be == hipSuccess) enqueue();
   const hipError_t le = hipGetLastError();
-  const hipError_t ee = hipStreamEndCapture(c->stream, &gr);
+  const hipError_t ee = be == hipSuccess ? hipStreamEndCapture(c->stream, &gr) : be;
+  c->timing = timing;
+  HIP_OK(be);
   HIP_OK(le);
   HIP_OK(ee);
+  size_t nn = 0;
+  hipError_t ne = hipGraphGetNodes(gr, nullptr, &nn);
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (ne == hipSuccess && nn) ne = hipGraphGetNodes(gr, nodes.data(), &nn);
+  bool kernels_only = ne == hipSuccess;
+  for (size_t i = 0; i < nn && kernels_only; ++i) {
+    hipGraphNodeType t;
+    kernels_only = hipGraphNodeGetType(nodes[i], &t) == hipSuccess && t == hipGraphNodeTypeKernel;
+  }
+  const PencilTune& tu = c->geom.tune;
+  const std::array<int, 7> key{kind, c->N, c->geom.gamma_mode, tu.a, tu.a0, tu.b, tu.c};
+  auto it = g_chunk_nodes.find(key);
+  const bool count_ok = it == g_chunk_nodes.end() || it->second == nn;
+  if (!kernels_only || !count_ok || nn == 0) {
+    (void)hipGraphDestroy(gr);
+    return fail(-101, "graph capture: the captured graph is not the chunk's kernel sequence");
+  }
+  if (it == g_chunk_nodes.end()) g_chunk_nodes.emplace(key, nn);
   const hipError_t ie = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gr);
   HIP_OK(ie);
+  HIP_OK(hipGraphUpload(*out, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -671,37 +712,46 @@ extern "C" int cpfft_set_tune(cpfft_ctx* c, const int in[4]) {
 // tangent_homo), captured now rather than on first use, so that a batch of
 // contexts run from several threads only launches graphs
 static int prepare_graphs(cpfft_ctx* c) {
-  if (!c->use_graphs) return 0;
+  if (c->use_graphs != 1) return 0;
   double* x = c->fields[CPFFT_DFM];
   bool have = false;
   for (auto& pg : c->pcg_graphs) have |= pg.first == x;
   int rc;
   if (!have) {
     hipGraphExec_t ge = nullptr;
-    if ((rc = capture_chunk(c, [&] { enqueue_pcg_chunk(c, x); }, &ge))) return rc;
+    if ((rc = capture_chunk(c, 0, [&] { enqueue_pcg_chunk(c, x); }, &ge))) return rc;
     c->pcg_graphs.emplace_back(x, ge);
   }
   if (!c->homo_graph &&
-      (rc = capture_chunk(c, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
+      (rc = capture_chunk(c, 1, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
     return rc;
+  return 0;
+}
+
+extern "C" int cpfft_set_cg_maxit(cpfft_ctx* c, int maxit) {
+  if (!c) return fail(-1, "cpfft_set_cg_maxit: null context");
+  if (maxit < 1) return fail(-1, "cpfft_set_cg_maxit: the limit must be positive");
+  c->cg_maxit = maxit;
   return 0;
 }
 
 extern "C" int cpfft_set_graphs(cpfft_ctx* c, int on) {
   if (!c) return fail(-1, "cpfft_set_graphs: null context");
-  c->use_graphs = on ? 1 : 0;
+  if (on < 0 || on > 2) return fail(-1, "cpfft_set_graphs: mode must be 0, 1 or 2");
+  c->use_graphs = on;
   return prepare_graphs(c);
 }
 
 // fftPcg (FFT_nr3.f:214-360): textbook unpreconditioned CG, x0 = 0
 extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, int* iters,
                          double* relres) {
+  if (!c) return fail(-1, "cpfft_pcg: null context");
   const long n = 9 * c->V;
-  const int maxIter = 1000;
+  const int maxIter = c->cg_maxit;
   const double eps = 2.220446049250313e-16;
   if (iters) *iters = 0;
   if (relres) *relres = 0.0;
-  launch_scal(n, 0.0, x, c->stream);
+  launch_fill(n, 0.0, x, c->stream);
   double bb;
   int rc = dot_host(c, b, b, &bb);
   if (rc) return rc;
@@ -729,7 +779,7 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
     for (auto& pg : c->pcg_graphs)
       if (pg.first == x) ge = pg.second;
     if (!ge) {
-      if ((rc = capture_chunk(c, [&] { enqueue_pcg_chunk(c, x); }, &ge))) return rc;
+      if ((rc = capture_chunk(c, 0, [&] { enqueue_pcg_chunk(c, x); }, &ge))) return rc;
       c->pcg_graphs.emplace_back(x, ge);
     }
   }
@@ -756,7 +806,7 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   c->gkdf_calls += k;
   if (iters) *iters = k;
   if (relres) *relres = c->hcg->resnorm / n2b;
-  if (c->hcg->done == 2) return fail(-2, "cpfft_pcg: no convergence within 1000 iterations");
+  if (c->hcg->done == 2) return fail(-2, "cpfft_pcg: no convergence within the iteration limit");
   return 0;
 }
 
@@ -777,8 +827,10 @@ static void mlp_predict(cpfft_ctx* c, int step) {
 }
 
 extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
-  long long before[8];
-  HIP_OK(hipMemcpyAsync(before, c->dcnt, sizeof before, hipMemcpyDeviceToHost, c->stream));
+  if (!c) return fail(-1, "cpfft_drive_eps_sig: null context");
+  long long* before = c->hcnt;
+  long long* after = c->hcnt + 8;
+  HIP_OK(hipMemcpyAsync(before, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
   tstart(c, 1);
   const float* ws = nullptr;
   if (c->mlp_on && iiter > 0 && c->nslip == 12) {
@@ -798,8 +850,7 @@ extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
                              c->fields[CPFFT_K4], nullptr, c->dcnt, c->stream);
   tstop(c, 1);
   HIP_OK(hipGetLastError());
-  long long after[8];
-  HIP_OK(hipMemcpyAsync(after, c->dcnt, sizeof after, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(after, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   c->des_calls++;
   return (int)(after[3] - before[3]);
@@ -877,8 +928,9 @@ static void enqueue_homo_chunk(cpfft_ctx* c) {
 }
 
 extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
+  if (!c) return fail(-1, "cpfft_tangent_homo: null context");
   const long V = c->V, n = 9 * V;
-  const int maxIter = 1000;
+  const int maxIter = c->cg_maxit;
   const double eps = 2.220446049250313e-16;
   if (tol <= eps || tol >= 1.0) return fail(-1, "cpfft_tangent_homo: improper tolerance");
   const bool fused = gkdf_pencil_supported(c->N);
@@ -911,14 +963,14 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
     h.maxit = maxIter;
     h.done = (n2b[i] <= eps) ? 1 : 0;  // zero right-hand side: dF_i = 0
     c->hcg9[i] = h;
-    if (!fused) launch_scal(n, 0.0, X[i], c->stream);
+    if (!fused) launch_fill(n, 0.0, X[i], c->stream);
     launch_copy(n, R[i], Pd[i], c->stream);
   }
   HIP_OK(hipMemcpyAsync(c->dcg9, c->hcg9, 9 * sizeof(CGScalars), hipMemcpyHostToDevice, c->stream));
-  if (fused) HIP_OK(hipMemsetAsync(c->hm, 0, 81 * sizeof(double), c->stream));
+  if (fused) launch_fill(81, 0.0, c->hm, c->stream);
   const bool graph = c->use_graphs && !c->timing;
   if (graph && !c->homo_graph &&
-      (rc = capture_chunk(c, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
+      (rc = capture_chunk(c, 1, [&] { enqueue_homo_chunk(c); }, &c->homo_graph)))
     return rc;
   for (;;) {
     if (graph) {
@@ -936,13 +988,13 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
   for (int i = 0; i < 9; ++i) {
     c->cg_iters += c->hcg9[i].k;
     c->gkdf_calls += c->hcg9[i].k;
-    if (c->hcg9[i].done == 2) return fail(-2, "cpfft_tangent_homo: CG did not converge within 1000 iterations");
+    if (c->hcg9[i].done == 2) return fail(-2, "cpfft_tangent_homo: CG did not converge within the iteration limit");
   }
   if (fused) {
     // C_homo(9(j-1)+i) = mean(K4 : E_i)_j + mean(K4 : x_i)_j, the first
     // term the mean of K4 component 9j+i, the second accumulated above
-    double hm[81];
-    HIP_OK(hipMemcpyAsync(hm, c->hm, sizeof hm, hipMemcpyDeviceToHost, c->stream));
+    double* hm = c->hhm;
+    HIP_OK(hipMemcpyAsync(hm, c->hm, 81 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     for (int j = 0; j < 9; ++j) {
       launch_mean9(K4 + (long)9 * j * V, V, c->partials, c->dres, c->stream);
       if ((rc = read_res(c, 9))) return rc;
@@ -968,8 +1020,10 @@ extern "C" int cpfft_update(cpfft_ctx* c) {
 }
 
 extern "C" int cpfft_counters(cpfft_ctx* c, long long out[8]) {
-  long long d[8];
-  HIP_OK(hipMemcpy(d, c->dcnt, sizeof d, hipMemcpyDeviceToHost));
+  if (!c) return fail(-1, "cpfft_counters: null context");
+  long long* d = c->hcnt;
+  HIP_OK(hipMemcpyAsync(d, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   out[0] = c->gkdf_calls;
   out[1] = c->cg_iters;
   out[2] = c->des_calls;

@@ -16,6 +16,7 @@
 //    block reductions (fixed order, no floating-point atomics).
 //  * drive_eps_sig (src/drive_eps_sig.f:16-339): one thread per voxel,
 //    body in mm10_core.h.
+#include "../../include/cpfft.h"
 #include "green.h"
 #include "kernels.h"
 
@@ -279,15 +280,10 @@ static constexpr int kRedBlocks = 1024;
 
 int reduce_partials_count(long) { return kRedBlocks * 9; }
 
+// halving-tree sum over a block of kBlock threads (tree_sum0), valid in
+// thread 0; the closing barrier lets a caller reuse sh in a loop
 __device__ inline double block_sum(double v, double* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (tid < s) sh[tid] += sh[tid + s];
-    __syncthreads();
-  }
-  double r = sh[0];
+  const double r = tree_sum0<kBlock>(v, sh);
   __syncthreads();
   return r;
 }
@@ -390,6 +386,20 @@ __global__ void k_fill9(long V, V9 v, double* __restrict__ x) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < 9 * V; i += (long)gridDim.x * blockDim.x)
     x[i] = v.v[i / V];
 }
+__global__ void k_fill(long n, double v, double* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void k_state_init(long V, int W, double tt0, double* __restrict__ st) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)W * V;
+       i += (long)gridDim.x * blockDim.x) {
+    const long w = i / V;
+    double v = 0.0;
+    if (w == CPFFT_W_RP + 0 || w == CPFFT_W_RP + 4 || w == CPFFT_W_RP + 8) v = 1.0;
+    if (w == CPFFT_W_TT) v = tt0;
+    st[i] = v;
+  }
+}
 __global__ void k_add1(long V, double* __restrict__ x) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < V; i += (long)gridDim.x * blockDim.x)
     x[i] = x[i] + 1.0;
@@ -440,6 +450,12 @@ void launch_fill9(long V, const double* v9, double* x, hipStream_t s) {
   V9 v;
   for (int i = 0; i < 9; ++i) v.v[i] = v9[i];
   k_fill9<<<grid_for(9 * V), kBlock, 0, s>>>(V, v, x);
+}
+void launch_fill(long n, double v, double* x, hipStream_t s) {
+  k_fill<<<grid_for(n), kBlock, 0, s>>>(n, v, x);
+}
+void launch_state_init(long V, int W, double tt0, double* state, hipStream_t s) {
+  k_state_init<<<grid_for((long)W * V), kBlock, 0, s>>>(V, W, tt0, state);
 }
 void launch_add_identity_comp(long V, int comp, double* x, hipStream_t s) {
   k_add1<<<grid_for(V), kBlock, 0, s>>>(V, x + (long)comp * V);

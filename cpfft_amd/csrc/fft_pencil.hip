@@ -672,13 +672,8 @@ __global__ __launch_bounds__(kT) void kD_zinv(const cplx* __restrict__ T1, doubl
   }
   if constexpr (DOT || UPD) {
     __shared__ double red[kT];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = kT / 2; h > 0; h >>= 1) {
-      if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = red[0];
+    const double sum = tree_sum0<kT>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = sum;
   }
 }
 

@@ -146,6 +146,11 @@ void launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);
 void launch_scal(long n, double a, double* x, hipStream_t s);
 void launch_copy(long n, const double* x, double* y, hipStream_t s);
 void launch_fill9(long V, const double* v9_host, double* x, hipStream_t s);
+// x[0..n) = v (a store, unlike a scaling by zero, which keeps NaN or Inf)
+void launch_fill(long n, double v, double* x, hipStream_t s);
+// initial material state [W][V] (mm10_init_cc_hist0, mm10_a.f:473-573, with
+// mm10_init_voche, :2042-2055): zero, except Rp = I and tau_tilde = tt0
+void launch_state_init(long V, int W, double tt0, double* state, hipStream_t s);
 void launch_add_identity_comp(long V, int comp, double* x, hipStream_t s);
 // CG vector update: x += a p; r -= a q; partial r.r into partials
 void launch_cg_update(long n, double a, const double* p, const double* q, double* x, double* r,

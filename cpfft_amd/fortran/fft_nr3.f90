@@ -129,6 +129,13 @@ contains
     Pbar_out = 0.0d0
     isNBC = isNBC_c /= 0
     existNBC = any(isNBC)
+    ! fftPcg aborts on a tolerance outside (eps, 1) (FFT_nr3.f:248-251);
+    ! checked here once, so that a continued run whose steps never reach
+    ! a CG solve with a nonzero right-hand side still reports it
+    if (tolPCG <= epsilon(1.0d0) .or. tolPCG >= 1.0d0) then
+      cpfft_compute = -1
+      return
+    end if
     Fn1 = cpfft_field(ctx, CPFFT_FN1)
     Pn1 = cpfft_field(ctx, CPFFT_PN1)
     dFm = cpfft_field(ctx, CPFFT_DFM)
@@ -195,8 +202,8 @@ contains
         rc = cpfft_dscal(ctx, -1.0d0, b)
         rc = cpfft_pcg(ctx, b, dFm, tolPCG, its, relres)
         times(1) = times(1) + (wall() - t0)
-        if (rc < -1) then
-          cpfft_compute = -2
+        if (rc < 0) then        ! rc = 1 (zero right-hand side, dFm = 0) goes on
+          cpfft_compute = rc
           return
         end if
         rc = cpfft_daxpy(ctx, 1.0d0, dFm, Fn1)
@@ -220,8 +227,8 @@ contains
           rc = cpfft_dscal(ctx, -1.0d0, b)
           rc = cpfft_pcg(ctx, b, dFm, tolPCG, its, relres)
           times(1) = times(1) + (wall() - t1)
-          if (rc < -1) then
-            cpfft_compute = -2
+          if (rc < 0) then
+            cpfft_compute = rc
             return
           end if
           rc = cpfft_daxpy(ctx, 1.0d0, dFm, Fn1)
@@ -305,11 +312,10 @@ contains
     rc = cpfft_sync(ctx)
     times(3) = wall() - times(3)
 
-1000 format(/,4x,'--------------------------------------------------', &
-         '-----------------------',/,5x,'Now starting step: ',i7)
-1001 format(7x,'Iteration ', i5, 7x, '  residual ', d10.3)
-1002 format(7x,'Stress iteration ',i5, '  residual ',d10.3)
-1003 format(7x,'Initial residual ', 16x, d10.3)
+1000 format(/,' load step ',i0)
+1001 format('   Newton ',i3,'  |dF|/|F| = ',es11.4)
+1002 format('   mixed-BC loop ',i3,'  |Pbar - P_BC| = ',es11.4)
+1003 format('   after the macroscopic increment  |dF|/|F| = ',es11.4)
   end function cpfft_compute
 
 end module cpfft_driver

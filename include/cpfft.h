@@ -158,12 +158,24 @@ int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
 int cpfft_f2disp(cpfft_ctx* ctx, const double* F, const double lengths[3], double* u);
 
 /* ---- execution options (no counterpart in the reference) -------------- */
-/* on != 0: cpfft_pcg and cpfft_tangent_homo replay their chunks of eight CG
- * iterations from HIP graphs captured on first use (one per solution
- * pointer for cpfft_pcg), one graph launch per host round trip instead of
- * about 70 kernel launches.  Results are unchanged.  While kernel timing is
- * on (cpfft_set_timing) the kernels are launched one by one.              */
+/* on = 1 or 2: cpfft_pcg and cpfft_tangent_homo replay their chunks of
+ * eight CG iterations from HIP graphs (one per solution pointer for
+ * cpfft_pcg), one graph launch per host round trip instead of about 70
+ * kernel launches.  Results are unchanged.  With 1 the graphs the load-step
+ * driver uses are captured now (and again after a change of launch shapes);
+ * with 2 each is captured on its first use.  Captures are serialised over
+ * the process and checked (kernel nodes only, the same count as the first
+ * capture of that chunk; -101 otherwise).  While kernel timing is on
+ * (cpfft_set_timing) the kernels are launched one by one.  0 turns graphs
+ * off; other values: -1.                                                  */
 int cpfft_set_graphs(cpfft_ctx* ctx, int on);
+
+/* Iteration limit of cpfft_pcg and of tangent_homo's solves, 1000 by
+ * default as fftPcg's maxIter (FFT_nr3.f:224).  A solve that needs the
+ * limit's number of iterations fails (-2) even when the last one passed
+ * the convergence test, as fftPcg aborts on itercount >= maxIter
+ * (FFT_nr3.f:335-339).  -1 for a limit below 1.                           */
+int cpfft_set_cg_maxit(cpfft_ctx* ctx, int maxit);
 
 /* Autotuner of the compile-time-planned FFT path (N in 16, 32, 48, 64, 96,
  * 128, 256): times the candidate launch shapes of each of pass A with K4,

@@ -101,7 +101,7 @@ void orc_ddot42n(long V, const double* A4, const double* B2, double* C2);
 /* fftPcg (FFT_nr3.f:214-360); returns 0 ok, 1 zero RHS, -1 bad tol, -2 maxit */
 int  orc_pcg(orc_fft* f, const double* K4, const double* b, double* x, double tol,
              int* iters, double* relres);
-void orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]);
+int  orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]);
 int  orc_nbc_update(const double C_homo[81], double DbarF[9], const double P_bar[9],
                     const double PBC[9], const int isNBC[9]);
 

@@ -271,8 +271,11 @@ int orc_pcg(orc_fft* f, const double* K4, const double* b, double* x, double tol
   return status;
 }
 
-/* tangent_homo (tangent_homo.f:11-73) */
-void orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]) {
+/* tangent_homo (tangent_homo.f:11-73).  Returns 0, or the first negative
+ * status of its nine fftPcg solves (fftPcg aborts there, FFT_nr3.f:248-251,
+ * 335-339); C_homo is filled either way. */
+int orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]) {
+  int status = 0;
   long V = f->V, n = 9 * V;
   double* C = (double*)malloc(n * sizeof(double));
   double* b = (double*)malloc(n * sizeof(double));
@@ -281,7 +284,8 @@ void orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81
     orc_gkdf(f, K4, C, b, 0);
     for (long a = 0; a < n; ++a) b[a] = -b[a];
     int it; double rel;
-    orc_pcg(f, K4, b, C, tol, &it, &rel);
+    int st = orc_pcg(f, K4, b, C, tol, &it, &rel);
+    if (st < 0 && status == 0) status = st;
     for (long e = 0; e < V; ++e) C[i * V + e] += 1.0;
     for (int j = 0; j < 9; ++j) {
       double s = dot(n, K4 + (long)9 * j * V, C);
@@ -289,6 +293,7 @@ void orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81
     }
   }
   free(C); free(b);
+  return status;
 }
 
 /* NBC_update (FFT_nr3.f:375-433) */
@@ -352,9 +357,10 @@ int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNB
   int existNBC = 0, rc = 0;
   for (int i = 0; i < 9; ++i) existNBC |= isNBC[i];
 
+  if (pb->tolPCG <= 2.220446049250313e-16 || pb->tolPCG >= 1.0) { rc = -1; goto done; }
   orc_drive_eps_sig(cp, V, pb->angles, 1, 0, pb->dt, pb->Fn, pb->Fn1, pb->sn, pb->snp1, pb->Pn1,
                     pb->K4, NULL);
-  orc_tangent_homo(f, pb->K4, pb->tolPCG, C_homo);
+  if ((rc = orc_tangent_homo(f, pb->K4, pb->tolPCG, C_homo)) < 0) goto done;
 
   for (int step = 1; step <= nstep; ++step) {
     int newton = 0, cgits = 0, nbc_loops = 0, calls = 0;
@@ -376,7 +382,8 @@ int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNB
       orc_gkdf(f, pb->K4, dFm, b, 1);
       for (long a = 0; a < n; ++a) b[a] = -b[a];
       int it; double rel;
-      if (orc_pcg(f, pb->K4, b, dFm, pb->tolPCG, &it, &rel) < -1) { rc = -2; goto done; }
+      if ((rc = orc_pcg(f, pb->K4, b, dFm, pb->tolPCG, &it, &rel)) < 0) goto done;
+      rc = 0;
       cgits += it;
       axpy(n, 1.0, dFm, pb->Fn1);
       int iiter_EBC = 0;
@@ -387,7 +394,8 @@ int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNB
         calls++;
         orc_gkdf(f, pb->K4, pb->Pn1, b, 0);
         for (long a = 0; a < n; ++a) b[a] = -b[a];
-        if (orc_pcg(f, pb->K4, b, dFm, pb->tolPCG, &it, &rel) < -1) { rc = -2; goto done; }
+        if ((rc = orc_pcg(f, pb->K4, b, dFm, pb->tolPCG, &it, &rel)) < 0) goto done;
+        rc = 0;
         cgits += it;
         axpy(n, 1.0, dFm, pb->Fn1);
         resfft = nrm2(n, dFm) / Fnorm;
@@ -411,7 +419,7 @@ int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNB
       nbc_loops++;
       if (res3 <= pb->tolNR) break;
       if (iiter_NBC > pb->maxIter) { rc = -5; goto done; }
-      orc_tangent_homo(f, pb->K4, pb->tolPCG, C_homo);
+      if ((rc = orc_tangent_homo(f, pb->K4, pb->tolPCG, C_homo)) < 0) goto done;
       for (int i = 0; i < 9; ++i) DbarF[i] = 0.0;
       if (orc_nbc_update(C_homo, DbarF, P_bar, PBC, isNBC)) { rc = -3; goto done; }
       for (int i = 0; i < 9; ++i) barF[i] = barF[i] + DbarF[i];

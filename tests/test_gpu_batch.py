@@ -91,7 +91,10 @@ def test_batch_matches_sequential_bitwise():
         assert rc == 0
         seq.append((Pbar, cnt))
         s.close()
-    for graphs in (False, True):
+    # graphs: 0 off, 1 captured before the batch starts, 2 captured lazily at
+    # first use while other threads run their RVEs (captures are serialised
+    # and each is checked to hold exactly the chunk's kernels)
+    for graphs in (0, 1, 2):
         solvers = []
         batch = []
         for wl, BC, isNBC in jobs:
