@@ -42,7 +42,8 @@ EXPORTS = [
     "cpfft_upload", "cpfft_download", "cpfft_reset", "cpfft_sync", "cpfft_driver_state",
     "cpfft_gkdf", "cpfft_pcg",
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
-    "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_f2disp", "cpfft_counters",
+    "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_f2disp",
+    "cpfft_gkdf_noalloc", "cpfft_counters",
     "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_set_cg_maxit", "cpfft_autotune",
     "cpfft_get_tune", "cpfft_set_tune",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
@@ -88,6 +89,7 @@ def lib():
         L.cpfft_field.restype = _vp
         L.cpfft_set_graphs.argtypes = [_vp, ctypes.c_int]
         L.cpfft_set_cg_maxit.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_gkdf_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]
         L.cpfft_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.cpfft_get_tune.argtypes = [_vp, _ip]
         L.cpfft_set_tune.argtypes = [_vp, _ip]

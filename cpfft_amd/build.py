@@ -17,7 +17,8 @@ CSRC = os.path.join(PKG, "csrc")
 FSRC = os.path.join(PKG, "fortran")
 BUILD = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["cpfft_kernels.hip", "fft_pencil.hip", "mlp.hip", "f2disp.hip", "cpfft_api.hip"]
+HIP_SOURCES = ["cpfft_kernels.hip", "fft_pencil.hip", "mlp.hip", "f2disp.hip", "cpfft_api.hip",
+               "cpfft_noalloc.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are disabled on this platform

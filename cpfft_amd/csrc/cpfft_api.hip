@@ -26,6 +26,8 @@ static int fail(int code, const char* what) {
   return code;
 }
 
+void cpfft::set_last_error(const char* what) { g_err = what; }
+
 #define HIP_OK(expr)                                                              \
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
@@ -178,6 +180,9 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(-2, "cpfft_create: no HIP device available (this library has no CPU path)");
+  // set-up of cpfft_gkdf_noalloc's scratch on this device, so that a later
+  // call there (perhaps under graph capture) only launches kernels
+  if (int rc = noalloc_scratch_current_device(nullptr)) return rc;
   cpfft_ctx* c = new cpfft_ctx();
   HIP_OK(hipGetDevice(&c->dev));
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -188,27 +193,9 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   // geometry
   int maxlds = 65536;
   (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->dev);
-  if (maxlds > 65536) maxlds = 65536;
-  SpecGeom& g = c->geom;
-  g.N = N;
-  g.Nz = c->Nz;
-  g.V = V;
-  g.S1 = (long)N * N * c->Nz;
-  g.plan = make_plan(N);
-  g.gamma_mode = gamma_mode;
-  g.ty = 8;
-  g.tz = 1;
-  while (g.tz < 4 && 2 * 9 * (long)(2 * g.tz) * N * (long)sizeof(cplx) <= maxlds) g.tz *= 2;
-  if (!gkdf_pencil_supported(N)) {  // limits of the generic runtime-plan path
-    if (2 * 9 * (long)N * (long)sizeof(cplx) > maxlds) {
-      cpfft_destroy(c);
-      return fail(-3, "cpfft_create: grid edge too large for the LDS of this device");
-    }
-    for (int s = 0; s < g.plan.nst; ++s)
-      if (g.plan.rad[s] > 16) {
-        cpfft_destroy(c);
-        return fail(-3, "cpfft_create: N has a prime factor above 16");
-      }
+  if (const char* why = make_spec_geom(N, gamma_mode, maxlds, c->geom)) {
+    cpfft_destroy(c);
+    return fail(-3, (std::string("cpfft_create: ") + why).c_str());
   }
   // material
   double b[24][3], nn[24][3];
@@ -246,7 +233,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
     c->field_len[f] = lens[f];
     HIP_OK(hipMalloc(&c->fields[f], lens[f] * sizeof(double)));
   }
-  HIP_OK(hipMalloc(&c->spec, 2 * 9 * g.S1 * sizeof(cplx)));
+  HIP_OK(hipMalloc(&c->spec, 2 * 9 * c->geom.S1 * sizeof(cplx)));
   HIP_OK(hipMalloc(&c->r, 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->p, 9 * V * sizeof(double)));
   HIP_OK(hipMalloc(&c->q, 9 * V * sizeof(double)));

@@ -259,6 +259,27 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_inv(SpecGeom g, const cplx* __
   }
 }
 
+const char* make_spec_geom(int N, int gamma_mode, int lds, SpecGeom& g) {
+  const int maxlds = lds > 65536 ? 65536 : lds;
+  g = SpecGeom{};
+  g.N = N;
+  g.Nz = N / 2 + 1;
+  g.V = (long)N * N * N;
+  g.S1 = (long)N * N * g.Nz;
+  g.plan = make_plan(N);
+  g.gamma_mode = gamma_mode;
+  g.ty = 8;
+  g.tz = 1;
+  while (g.tz < 4 && 2 * 9 * (long)(2 * g.tz) * N * (long)sizeof(cplx) <= maxlds) g.tz *= 2;
+  if (!gkdf_pencil_supported(N)) {  // limits of the generic runtime-plan path
+    if (2 * 9 * (long)N * (long)sizeof(cplx) > maxlds)
+      return "grid edge too large for the LDS of this device";
+    for (int s = 0; s < g.plan.nst; ++s)
+      if (g.plan.rad[s] > 16) return "N has a prime factor above 16";
+  }
+  return nullptr;
+}
+
 void launch_gkdf(const SpecGeom& g, const double* F, const double* K4, double* out, cplx* spec,
                  const cplx* tw, hipStream_t s) {
   if (launch_gkdf_pencil(g, F, K4, out, spec, tw, s)) return;

@@ -33,6 +33,13 @@ struct SpecGeom {
   PencilTune tune;  // launch shapes of the planned path
 };
 
+// Geometry of the operator for edge N and Gamma variant gamma_mode: the
+// spectrum sizes, the radix plan and the generic path's tiles, sized for
+// min(lds, 64 KiB) bytes of shared memory per block (the same tiles on
+// every device, so results do not depend on the device).  Returns nullptr,
+// or why N cannot be served.
+const char* make_spec_geom(int N, int gamma_mode, int lds, SpecGeom& g);
+
 // Displacement reconstruction (f2disp.hip): the cosine table (of
 // f2disp_cos_doubles(N) doubles) once, then u (device, [3 (N+1)^3]
 // node-major) from F; work holds f2disp_scratch_doubles(N) doubles and must
@@ -184,5 +191,12 @@ void launch_mlp_denorm(long V, long e0, long nrows, const float* Y, const float*
                        hipStream_t s);
 
 int reduce_partials_count(long n);
+
+// error text for cpfft_last_error (of the calling thread)
+void set_last_error(const char* what);
+// cpfft_gkdf_noalloc's scratch: the device address of the current GPU's
+// copy of its static array (resolved on the first call per device; also
+// from cpfft_create).  0, or a negative code with the error text set.
+int noalloc_scratch_current_device(cplx** out);
 
 }  // namespace cpfft

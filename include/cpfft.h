@@ -157,6 +157,27 @@ int cpfft_mean9(cpfft_ctx* ctx, const double* x, double out[9]);
  * argument.                                                                */
 int cpfft_f2disp(cpfft_ctx* ctx, const double* F, const double lengths[3], double* u);
 
+/* ---- G_K_dF for callers that cannot allocate --------------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Computes G_K_dF exactly as cpfft_gkdf
+ * does (the same kernels; bit-identical results) without a context:
+ * Gamma:(K4:F), or Gamma:F when K4 is NULL, for the N^3 grid, on
+ * caller-owned device arrays F and GKF ([9][N^3], may not alias) and K4
+ * ([81][N^3]), enqueued on `stream` (a hipStream_t; NULL: the null stream)
+ * and returning without waiting.  The spectrum scratch is one static
+ * device array of 16.6 MB per GPU (N <= CPFFT_NOALLOC_MAX_N), held until
+ * the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU (different streams or threads) race, so callers
+ * serialise them.  Its address is resolved once per device (by
+ * cpfft_create, or by the first call there); the call itself makes no
+ * allocation.  Returns 0; -1 for a bad argument, N above the maximum
+ * included; -3 for an N the operator cannot serve (prime factor above 16);
+ * -100 or below for a HIP error (cpfft_last_error() describes each).     */
+#define CPFFT_NOALLOC_MAX_N 48
+int cpfft_gkdf_noalloc(int N, int gamma_mode, const double* F, const double* K4, double* GKF,
+                       void* stream);
+
 /* ---- execution options (no counterpart in the reference) -------------- */
 /* on = 1 or 2: cpfft_pcg and cpfft_tangent_homo replay their chunks of
  * eight CG iterations from HIP graphs (one per solution pointer for
