@@ -119,7 +119,7 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
 def workload_text(wl):
     """Crystal and loading of a workload, e.g. 'FCC cubic Voce n=20, uniaxial
     tension 1e-3/s'."""
-    crystal = "FCC" if wl.material.slip_set == 1 else "BCC24"
+    crystal = {1: "FCC", 8: "BCC48", 80: "BCC24"}[wl.material.slip_set]
     span = wl.strain * (4 * wl.cycles if wl.cycles > 0 else 1)  # strain travelled
     rate = f"{span / (wl.nstep * wl.dt):.0e}".replace("e-0", "e-")
     load = (f"uniaxial tension-compression +-{100 * wl.strain:g}% x{wl.cycles}, {rate}/s"

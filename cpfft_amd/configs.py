@@ -32,7 +32,7 @@ NU_CUBIC = C12 / (C11 + C12)
 
 @dataclass
 class Material:
-    slip_set: int = 1           # 1 fcc, 8 bcc {110}+{112}
+    slip_set: int = 1           # 1 fcc, 8 bcc48, 80 bcc24 (bcc48's {110} and {112} systems)
     e: float = E_CUBIC
     nu: float = NU_CUBIC
     mu: float = C44
@@ -109,7 +109,7 @@ WORKLOADS = {
     # BCC (the first 24 systems of bcc48), cyclic +-1.5%, 3 cycles; the cubic
     # elastic constants of the FCC material are kept
     "c5_256": Workload("c5_256", 256, 1024, 20261202 + 5, 720, 0.015, cycles=3,
-                       material=Material(slip_set=8)),
+                       material=Material(slip_set=80)),
 }
 
 

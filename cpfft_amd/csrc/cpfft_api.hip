@@ -115,7 +115,8 @@ struct cpfft_ctx {
 
 template <int NS>
 static int upload_orients(cpfft_ctx* c, const std::vector<std::array<double, 3>>& uniq,
-                          const double stiff[6][6], const double b[24][3], const double n[24][3]) {
+                          const double stiff[6][6], const double b[kMaxSlip][3],
+                          const double n[kMaxSlip][3]) {
   std::vector<Orient<NS>> h(uniq.size());
   for (size_t k = 0; k < uniq.size(); ++k) build_orient<NS>(stiff, b, n, uniq[k].data(), h[k]);
   HIP_OK(hipMalloc(&c->orients, h.size() * sizeof(Orient<NS>)));
@@ -174,7 +175,8 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   *out = nullptr;
   if (N < 2 || N > 256) return fail(-1, "cpfft_create: N must be in [2, 256]");
   if (!m || !angles) return fail(-1, "cpfft_create: null material or angles");
-  if (m->slip_set != 1 && m->slip_set != 8) return fail(-1, "cpfft_create: slip_set must be 1 (fcc) or 8 (bcc24)");
+  if (slip_count(m->slip_set) == 0)
+    return fail(-1, "cpfft_create: slip_set must be 1 (fcc), 8 (bcc48) or 80 (bcc24)");
   if (gamma_mode != 0 && gamma_mode != 1) return fail(-1, "cpfft_create: bad gamma_mode");
   if (!(dt > 0.0)) return fail(-1, "cpfft_create: dt must be positive");
   int ndev = 0;
@@ -198,7 +200,7 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
     return fail(-3, (std::string("cpfft_create: ") + why).c_str());
   }
   // material
-  double b[24][3], nn[24][3];
+  double b[kMaxSlip][3], nn[kMaxSlip][3];
   slip_systems(m->slip_set, c->nslip, b, nn);
   double st[6][6];
   if (!cubic_stiffness(*m, st)) {
@@ -222,7 +224,9 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
       hoid[e] = it->second;
     }
   }
-  int rc = c->nslip == 12 ? upload_orients<12>(c, uniq, st, b, nn) : upload_orients<24>(c, uniq, st, b, nn);
+  int rc = c->nslip == 12   ? upload_orients<12>(c, uniq, st, b, nn)
+           : c->nslip == 24 ? upload_orients<24>(c, uniq, st, b, nn)
+                            : upload_orients<48>(c, uniq, st, b, nn);
   if (rc) { cpfft_destroy(c); return rc; }
   HIP_OK(hipMalloc(&c->oid, V * sizeof(int32_t)));
   HIP_OK(hipMemcpy(c->oid, hoid.data(), V * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -830,8 +834,13 @@ extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
                              c->fields[CPFFT_K4], ws, c->dcnt, c->stream);
-  else
+  else if (c->nslip == 24)
     launch_drive_eps_sig<24>(c->V, c->mp, (const Orient<24>*)c->orients, c->oid, step, iiter,
+                             c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                             c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
+                             c->fields[CPFFT_K4], nullptr, c->dcnt, c->stream);
+  else
+    launch_drive_eps_sig<48>(c->V, c->mp, (const Orient<48>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
                              c->fields[CPFFT_K4], nullptr, c->dcnt, c->stream);

@@ -825,5 +825,9 @@ template void launch_drive_eps_sig<24>(long, const MatParams&, const Orient<24>*
                                        int, int, const double*, const double*, const double*,
                                        double*, double*, double*, double*, const float*,
                                        long long*, hipStream_t);
+template void launch_drive_eps_sig<48>(long, const MatParams&, const Orient<48>*, const int32_t*,
+                                       int, int, const double*, const double*, const double*,
+                                       double*, double*, double*, double*, const float*,
+                                       long long*, hipStream_t);
 
 }  // namespace cpfft

@@ -13,35 +13,51 @@
 namespace cpfft {
 
 // ------------------------------------------------------------------ setup
-// crystal-frame slip systems: mod_crystals.f:438-515 (fcc) and the first 24
-// systems of bcc48, mod_crystals.f:812-1207 ({110} then {112})
-inline void slip_systems(int set, int& ns, double b[24][3], double n[24][3]) {
-  if (set == 1) {
-    static const int bb[12][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}, {0, 1, 1},  {1, 0, 1},  {1, -1, 0},
-                                  {0, -1, 1}, {-1, 0, -1}, {1, 1, 0}, {0, 1, 1},  {1, 0, -1}, {-1, -1, 0}};
-    static const int nn[12][3] = {{1, 1, 1},   {1, 1, 1},   {1, 1, 1},   {-1, -1, 1}, {-1, -1, 1}, {-1, -1, 1},
-                                  {-1, 1, 1},  {-1, 1, 1},  {-1, 1, 1},  {1, -1, 1},  {1, -1, 1},  {1, -1, 1}};
-    ns = 12;
-    const double f2 = 1 / sqrt(2.0), f3 = 1 / sqrt(3.0);
-    for (int s = 0; s < 12; ++s)
-      for (int k = 0; k < 3; ++k) { b[s][k] = bb[s][k] * f2; n[s][k] = nn[s][k] * f3; }
-  } else {
-    static const int bb[24][3] = {{-1, 1, 1}, {1, -1, 1}, {1, 1, 1},  {1, 1, -1}, {1, 1, -1}, {-1, 1, 1},
-                                  {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, -1, 1}, {1, 1, 1},  {-1, 1, 1},
-                                  {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1},
-                                  {1, 1, 1},  {-1, 1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1}};
-    static const int nn[24][3] = {{1, 1, 0},  {1, 1, 0},  {1, -1, 0}, {1, -1, 0}, {1, 0, 1},  {1, 0, 1},
-                                  {1, 0, -1}, {1, 0, -1}, {0, 1, 1},  {0, 1, 1},  {0, 1, -1}, {0, 1, -1},
-                                  {1, 1, 2},  {-1, 1, 2}, {1, -1, 2}, {1, 1, -2}, {1, 2, 1},  {-1, 2, 1},
-                                  {1, -2, 1}, {1, 2, -1}, {2, 1, 1},  {-2, 1, 1}, {2, -1, 1}, {2, 1, -1}};
-    ns = 24;
-    const double f2 = 1.0 / sqrt(2.0), f3 = 1.0 / sqrt(3.0), f112 = 1.0 / sqrt(6.0), f211 = 2.0 / sqrt(6.0);
-    for (int s = 0; s < 24; ++s)
-      for (int k = 0; k < 3; ++k) {
-        b[s][k] = bb[s][k] * f3;
-        int v = nn[s][k];
-        n[s][k] = s < 12 ? v * f2 : (v == 2 ? f211 : v == -2 ? -f211 : v * f112);
-      }
+// Crystal-frame slip systems as integer Miller triples: fcc
+// (mod_crystals.f:438-515) and bcc48 (mod_crystals.f:812-1207: {110}<111>,
+// {112}<111>, {123}<111>, in that order).  A unit vector has components
+// sign(v) |v|/sqrt(|v|^2), the reference's named factors (f3 = 1/sqrt(3),
+// f211 = 2/sqrt(6), f312 = 3/sqrt(14), ...) to the bit.
+constexpr int kMaxSlip = 48;
+enum { SLIP_FCC = 1, SLIP_BCC48 = 8, SLIP_BCC24 = 80 };
+
+inline int slip_count(int set) {
+  return set == SLIP_FCC ? 12 : set == SLIP_BCC48 ? 48 : set == SLIP_BCC24 ? 24 : 0;
+}
+
+inline void unit_miller(const int v[3], double out[3]) {
+  const double m = (double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int k = 0; k < 3; ++k) {
+    const double a = (double)(v[k] < 0 ? -v[k] : v[k]) / sqrt(m);
+    out[k] = v[k] < 0 ? -a : (v[k] > 0 ? a : 0.0);
+  }
+}
+
+// set: SLIP_FCC (12 systems), SLIP_BCC48 (48) or SLIP_BCC24 (the first 24
+// systems of bcc48, {110} and {112}); ns = 0 for another set
+inline void slip_systems(int set, int& ns, double b[kMaxSlip][3], double n[kMaxSlip][3]) {
+  static const int fb[12][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}, {0, 1, 1},  {1, 0, 1},  {1, -1, 0},
+                                {0, -1, 1}, {-1, 0, -1}, {1, 1, 0}, {0, 1, 1},  {1, 0, -1}, {-1, -1, 0}};
+  static const int fn[12][3] = {{1, 1, 1},  {1, 1, 1},  {1, 1, 1},  {-1, -1, 1}, {-1, -1, 1}, {-1, -1, 1},
+                                {-1, 1, 1}, {-1, 1, 1}, {-1, 1, 1}, {1, -1, 1},  {1, -1, 1},  {1, -1, 1}};
+  static const int bb[48][3] = {
+      {-1, 1, 1}, {1, -1, 1}, {1, 1, 1},  {1, 1, -1}, {1, 1, -1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1},
+      {1, 1, -1}, {1, -1, 1}, {1, 1, 1},  {-1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},
+      {1, -1, 1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1},
+      {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
+      {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
+      {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1}};
+  static const int bn[48][3] = {
+      {1, 1, 0},  {1, 1, 0},  {1, -1, 0}, {1, -1, 0}, {1, 0, 1},  {1, 0, 1},  {1, 0, -1}, {1, 0, -1},
+      {0, 1, 1},  {0, 1, 1},  {0, 1, -1}, {0, 1, -1}, {1, 1, 2},  {-1, 1, 2}, {1, -1, 2}, {1, 1, -2},
+      {1, 2, 1},  {-1, 2, 1}, {1, -2, 1}, {1, 2, -1}, {2, 1, 1},  {-2, 1, 1}, {2, -1, 1}, {2, 1, -1},
+      {1, 2, 3},  {-1, 2, 3}, {1, -2, 3}, {1, 2, -3}, {1, 3, 2},  {-1, 3, 2}, {1, -3, 2}, {1, 3, -2},
+      {2, 1, 3},  {-2, 1, 3}, {2, -1, 3}, {2, 1, -3}, {2, 3, 1},  {-2, 3, 1}, {2, -3, 1}, {2, 3, -1},
+      {3, 1, 2},  {-3, 1, 2}, {3, -1, 2}, {3, 1, -2}, {3, 2, 1},  {-3, 2, 1}, {3, -2, 1}, {3, 2, -1}};
+  ns = slip_count(set);
+  for (int s = 0; s < ns; ++s) {
+    unit_miller(set == SLIP_FCC ? fb[s] : bb[s], b[s]);
+    unit_miller(set == SLIP_FCC ? fn[s] : bn[s], n[s]);
   }
 }
 
@@ -87,7 +103,7 @@ inline void kocks_g(const double a[3], double g[3][3]) {
 }
 
 template <int NS>
-inline void build_orient(const double stiff[6][6], const double b[24][3], const double n[24][3],
+inline void build_orient(const double stiff[6][6], const double b[kMaxSlip][3], const double n[kMaxSlip][3],
                          const double ang[3], Orient<NS>& o) {
   double g[3][3], t[3][3];
   kocks_g(ang, g);

@@ -42,7 +42,9 @@ typedef struct cpfft_ctx cpfft_ctx;
  * Mirrors the crystal properties the deck command `crystal` sets
  * (src/incrystal.f, defaults src/mod_crystals.f:231-410). */
 typedef struct {
-  int32_t slip_set;   /* 1 = fcc {111}<110> (12), 8 = bcc {110}+{112} <111> (24) */
+  int32_t slip_set;   /* 1 = fcc {111}<110> (12); 8 = bcc48, {110}+{112}+{123}
+                         <111> (48), as the reference's slip_type 8; 80 = the
+                         first 24 systems of bcc48 ({110} and {112})        */
   double e, nu, mu;   /* cubic elasticity through the compliance (mod_crystals.f:1833-1879) */
   double rate_n;      /* harden_n: power-law rate exponent                   */
   double tau_y, tau_v, theta_0, voche_m;   /* Voce hardening                 */

@@ -53,7 +53,7 @@ typedef struct {
 #define ORC_NWORDS(ns)    (26 + 2 * (ns))
 
 /* ---- crystal setup ---------------------------------------------------- */
-/* slip_set: 1 = fcc (12), 8 = first 24 systems of bcc48 ({110}+{112}) */
+/* slip_set: 1 = fcc (12), 8 = bcc48 (48), 80 = first 24 systems of bcc48 ({110}+{112}) */
 int  orc_crystal_init(orc_crystal* c, int slip_set, double e, double nu, double mu,
                       double rate_n, double tau_y, double tau_v, double theta_0,
                       double voche_m, int alter_mode, double gamma_bar);

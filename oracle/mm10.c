@@ -237,42 +237,53 @@ int orc_crystal_init(orc_crystal* c, int slip_set, double e, double nu, double m
                      double rate_n, double tau_y, double tau_v, double theta_0,
                      double voche_m, int alter_mode, double gamma_bar) {
   memset(c, 0, sizeof *c);
-  /* slip systems: mod_crystals.f:438-515 (fcc), 812-1207 first 24 (bcc48) */
-  if (slip_set == 1) {
-    static const int b[12][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}, {0, 1, 1},
-                                 {1, 0, 1},  {1, -1, 0}, {0, -1, 1}, {-1, 0, -1},
-                                 {1, 1, 0},  {0, 1, 1},  {1, 0, -1}, {-1, -1, 0}};
-    static const int n[12][3] = {{1, 1, 1},  {1, 1, 1},  {1, 1, 1},  {-1, -1, 1},
-                                 {-1, -1, 1}, {-1, -1, 1}, {-1, 1, 1}, {-1, 1, 1},
-                                 {-1, 1, 1}, {1, -1, 1},  {1, -1, 1}, {1, -1, 1}};
-    double f2 = 1 / sqrt(2.0), f3 = 1 / sqrt(3.0);
-    c->nslip = 12;
-    for (int s = 0; s < 12; ++s)
-      for (int k = 0; k < 3; ++k) { c->bi[s][k] = b[s][k] * f2; c->ni[s][k] = n[s][k] * f3; }
-  } else if (slip_set == 8) {
-    static const int b[24][3] = {{-1, 1, 1}, {1, -1, 1}, {1, 1, 1},  {1, 1, -1}, {1, 1, -1},
-                                 {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, -1, 1},
-                                 {1, 1, 1},  {-1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1},
-                                 {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
-                                 {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1}};
-    /* normals: 1..12 {110} (entries are multiples of 1/sqrt2),
-     * 13..24 {112} (entries are multiples of 1/sqrt6) */
-    static const int n[24][3] = {{1, 1, 0},  {1, 1, 0},  {1, -1, 0}, {1, -1, 0}, {1, 0, 1},
-                                 {1, 0, 1},  {1, 0, -1}, {1, 0, -1}, {0, 1, 1},  {0, 1, 1},
-                                 {0, 1, -1}, {0, 1, -1}, {1, 1, 2},  {-1, 1, 2}, {1, -1, 2},
-                                 {1, 1, -2}, {1, 2, 1},  {-1, 2, 1}, {1, -2, 1}, {1, 2, -1},
-                                 {2, 1, 1},  {-2, 1, 1}, {2, -1, 1}, {2, 1, -1}};
-    double f2 = 1.0 / sqrt(2.0), f3 = 1.0 / sqrt(3.0), f112 = 1.0 / sqrt(6.0),
-           f211 = 2.0 / sqrt(6.0);
-    c->nslip = 24;
-    for (int s = 0; s < 24; ++s)
+  /* slip systems: mod_crystals.f:438-515 (slip_type 1, fcc) and 812-1207
+   * (slip_type 8, bcc48: {110}<111>, {112}<111>, {123}<111>); slip set 80
+   * is the first 24 systems of bcc48.  The reference assigns each entry a
+   * named factor (f = 1/sqrt(2), f3 = 1/sqrt(3), f211 = 2/sqrt(6),
+   * f312 = 3/sqrt(14), ...): component v of a Miller triple of squared length
+   * m becomes |v|/sqrt(m) with v's sign, which is that factor to the bit. */
+  static const int fb[12][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}, {0, 1, 1},
+                                {1, 0, 1},  {1, -1, 0}, {0, -1, 1}, {-1, 0, -1},
+                                {1, 1, 0},  {0, 1, 1},  {1, 0, -1}, {-1, -1, 0}};
+  static const int fn[12][3] = {{1, 1, 1},  {1, 1, 1},  {1, 1, 1},  {-1, -1, 1},
+                                {-1, -1, 1}, {-1, -1, 1}, {-1, 1, 1}, {-1, 1, 1},
+                                {-1, 1, 1}, {1, -1, 1},  {1, -1, 1}, {1, -1, 1}};
+  /* bcc48 directions <111> and normals: 1..12 {110}, 13..24 {112},
+   * 25..48 {123} */
+  static const int bb[48][3] = {
+      {-1, 1, 1}, {1, -1, 1}, {1, 1, 1},  {1, 1, -1}, {1, 1, -1}, {-1, 1, 1},
+      {1, 1, 1},  {1, -1, 1}, {1, 1, -1}, {1, -1, 1}, {1, 1, 1},  {-1, 1, 1},
+      {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1},
+      {1, 1, 1},  {-1, 1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1},
+      {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, -1, 1}, {1, 1, -1},
+      {1, 1, 1},  {-1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},
+      {1, -1, 1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1}, {-1, 1, 1}, {1, 1, 1},
+      {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1},  {1, 1, -1}, {1, -1, 1}};
+  static const int bn[48][3] = {
+      {1, 1, 0},  {1, 1, 0},  {1, -1, 0}, {1, -1, 0}, {1, 0, 1},  {1, 0, 1},
+      {1, 0, -1}, {1, 0, -1}, {0, 1, 1},  {0, 1, 1},  {0, 1, -1}, {0, 1, -1},
+      {1, 1, 2},  {-1, 1, 2}, {1, -1, 2}, {1, 1, -2}, {1, 2, 1},  {-1, 2, 1},
+      {1, -2, 1}, {1, 2, -1}, {2, 1, 1},  {-2, 1, 1}, {2, -1, 1}, {2, 1, -1},
+      {1, 2, 3},  {-1, 2, 3}, {1, -2, 3}, {1, 2, -3}, {1, 3, 2},  {-1, 3, 2},
+      {1, -3, 2}, {1, 3, -2}, {2, 1, 3},  {-2, 1, 3}, {2, -1, 3}, {2, 1, -3},
+      {2, 3, 1},  {-2, 3, 1}, {2, -3, 1}, {2, 3, -1}, {3, 1, 2},  {-3, 1, 2},
+      {3, -1, 2}, {3, 1, -2}, {3, 2, 1},  {-3, 2, 1}, {3, -2, 1}, {3, 2, -1}};
+  const int(*tb)[3], (*tn)[3];
+  if (slip_set == 1) { c->nslip = 12; tb = fb; tn = fn; }
+  else if (slip_set == 8) { c->nslip = 48; tb = bb; tn = bn; }
+  else if (slip_set == 80) { c->nslip = 24; tb = bb; tn = bn; }
+  else return -1;
+  for (int s = 0; s < c->nslip; ++s) {
+    const int* v[2] = {tb[s], tn[s]};
+    double* o[2] = {c->bi[s], c->ni[s]};
+    for (int a = 0; a < 2; ++a) {
+      double m = (double)(v[a][0] * v[a][0] + v[a][1] * v[a][1] + v[a][2] * v[a][2]);
       for (int k = 0; k < 3; ++k) {
-        c->bi[s][k] = b[s][k] * f3;
-        if (s < 12) c->ni[s][k] = n[s][k] * f2;
-        else c->ni[s][k] = (abs(n[s][k]) == 2) ? (n[s][k] > 0 ? f211 : -f211) : n[s][k] * f112;
+        double mag = (double)abs(v[a][k]) / sqrt(m);
+        o[a][k] = v[a][k] < 0 ? -mag : (v[a][k] > 0 ? mag : zero);
       }
-  } else {
-    return -1;
+    }
   }
   /* cubic elasticity via the compliance (mod_crystals.f:1833-1879) */
   double fl[6][6];

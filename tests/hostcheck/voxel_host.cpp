@@ -12,7 +12,7 @@ template <int NS>
 static void run(const cpfft_material& m, double dt, long V, const double* ang, int step,
                 int iter, const double* Fn, const double* Fn1, const double* sn, double* snp1,
                 double* Pn1, double* K4, long long* cnt, const float* ws = nullptr) {
-  double b[24][3], nn[24][3], st[6][6];
+  double b[kMaxSlip][3], nn[kMaxSlip][3], st[6][6];
   int ns = 0;
   slip_systems(m.slip_set, ns, b, nn);
   cubic_stiffness(m, st);
@@ -49,10 +49,12 @@ extern "C" int cpfft_host_drive_eps_sig(const cpfft_material* m, double dt, long
                                         const double* angles, int step, int iter,
                                         const double* Fn, const double* Fn1, const double* sn,
                                         double* snp1, double* Pn1, double* K4, long long* cnt) {
-  if (m->slip_set == 1)
-    run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt);
-  else
-    run<24>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt);
+  switch (slip_count(m->slip_set)) {
+    case 12: run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt); break;
+    case 24: run<24>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt); break;
+    case 48: run<48>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt); break;
+    default: return -1;
+  }
   return 0;
 }
 
@@ -62,7 +64,53 @@ extern "C" int cpfft_host_drive_eps_sig_ws(const cpfft_material* m, double dt, l
                                            const double* Fn, const double* Fn1, const double* sn,
                                            double* snp1, double* Pn1, double* K4, long long* cnt,
                                            const float* ws) {
-  if (m->slip_set != 1) return -1;
+  if (m->slip_set != SLIP_FCC) return -1;
   run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt, ws);
   return 0;
+}
+
+// the library's crystal-frame slip table for slip set `set`: b[s][3] and
+// n[s][3] for s < ns (48 at most); returns ns (0 for an unknown set)
+extern "C" int cpfft_host_slip_systems(int set, double* b, double* n) {
+  double bb[kMaxSlip][3], nn[kMaxSlip][3];
+  int ns = 0;
+  slip_systems(set, ns, bb, nn);
+  for (int s = 0; s < ns; ++s)
+    for (int k = 0; k < 3; ++k) {
+      b[3 * s + k] = bb[s][k];
+      n[3 * s + k] = nn[s][k];
+    }
+  return ns;
+}
+
+// the library's per-orientation invariants for Kocks angles ang (degrees):
+// rotated stiffness C[36] (row-major Voigt, engineering shear), Schmid
+// tensors ms[s][6] and spin vectors qs[s][3]; returns ns, or -1 when the
+// compliance is singular
+template <int NS>
+static void orient_out(const double st[6][6], const double b[kMaxSlip][3],
+                       const double n[kMaxSlip][3], const double* ang, double* C, double* ms,
+                       double* qs) {
+  Orient<NS> o;
+  build_orient<NS>(st, b, n, ang, o);
+  for (int k = 0; k < 36; ++k) C[k] = o.C[k];
+  for (int s = 0; s < NS; ++s) {
+    for (int k = 0; k < 6; ++k) ms[6 * s + k] = o.ms[s][k];
+    for (int k = 0; k < 3; ++k) qs[3 * s + k] = o.qs[s][k];
+  }
+}
+
+extern "C" int cpfft_host_orientation(const cpfft_material* m, const double* ang, double* C,
+                                      double* ms, double* qs) {
+  double b[kMaxSlip][3], nn[kMaxSlip][3], st[6][6];
+  int ns = 0;
+  slip_systems(m->slip_set, ns, b, nn);
+  if (!cubic_stiffness(*m, st)) return -1;
+  switch (ns) {
+    case 12: orient_out<12>(st, b, nn, ang, C, ms, qs); break;
+    case 24: orient_out<24>(st, b, nn, ang, C, ms, qs); break;
+    case 48: orient_out<48>(st, b, nn, ang, C, ms, qs); break;
+    default: return 0;
+  }
+  return ns;
 }

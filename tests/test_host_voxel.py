@@ -75,11 +75,16 @@ def check(a, b, nslip, what):
         assert rel(sn[w0:w1], so[w0:w1]) < 1e-9, (what, w0)
 
 
-@pytest.mark.parametrize("slip_set,jac_exact", [(1, 0), (1, 1), (8, 1)])
+# seed offset per slip set: 1 fcc, 80 bcc24 (numbered 8 when its seeds were
+# chosen), 8 bcc48
+SEED_OF = {1: 1, 80: 8, 8: 48}
+
+
+@pytest.mark.parametrize("slip_set,jac_exact", [(1, 0), (1, 1), (80, 1), (8, 0)])
 def test_host_update_matches_oracle(slip_set, jac_exact):
     V = 6 ** 3
-    rng = np.random.default_rng(40 + slip_set + jac_exact)
-    _, ang = polycrystal(6, 5, 11 + slip_set)
+    rng = np.random.default_rng(40 + SEED_OF[slip_set] + jac_exact)
+    _, ang = polycrystal(6, 5, 11 + SEED_OF[slip_set])
     mat = C.Material(slip_set=slip_set, jac_exact=jac_exact)
     oc = O.crystal(slip_set=slip_set, jac_exact=jac_exact)
     dt = 0.25

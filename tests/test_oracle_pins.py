@@ -1,0 +1,415 @@
+"""Independent pins of the oracle's crystal data, flow rule, residual and
+load-step loop (and of the library's host-side set-up, which parity with the
+oracle cannot pin when both read the same table the same way).
+
+* Slip tables: the oracle's and the library's crystal-frame tables equal
+  tests/golden/slip_tables.json, which make_slip_tables.py extracts
+  mechanically from the reference's assignments (mod_crystals.f:438-515,
+  812-1207); and independently of the reference text they are complete
+  crystallographic families: unit vectors, b . n = 0, fcc {111}<110> (12),
+  bcc {110}<111> (12), {112}<111> (12), {123}<111> (24), no system twice.
+* Stiffness: inverting the compliance of the cubic inputs (e, nu, mu)
+  returns C11/C12/C44 = 168.4/121.4/75.4 GPa; isotropic inputs return the
+  Lame form; the library's rotated stiffness, Schmid tensors and spin
+  vectors equal a fourth/second-order tensor rotation done in numpy.
+* Flow rule and residual: at the oracle's converged plastic state (tight
+  local tolerances), the stored slip increments equal the power law
+  dg |tau/g|^(n-1) tau/g of the stored stress and slip resistance, and an
+  independent numpy R1 (tensor-form kinematics, stiffness rotation,
+  Schmid-tensor and spin rotations, mm10_formR1 of mm10_b.f:1065-1113)
+  vanishes, in the first load step (Rp_n = I) and in a second one from the
+  stored state (Rp_n != I, which exercises the rotation of the Schmid
+  tensors by Rp_n, mm10_a.f:830-962).
+* Load-step loop: for a homogeneous single crystal the FFT solution is
+  uniform, so the oracle's FFT_nr3 mean stress per step must equal a
+  single-voxel update driven by a numpy Newton solve of the mixed boundary
+  condition (F_xx prescribed, P_yy = P_zz = 0).
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cpfft_amd.microstructure import kocks_matrix
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLES = json.load(open(os.path.join(HERE, "golden", "slip_tables.json")))
+I3 = np.eye(3)
+VI = [[0, 3, 5], [3, 1, 4], [5, 4, 2]]  # Voigt index of tensor (i, j)
+
+
+def unit_rows(a):
+    a = np.asarray(a, float)
+    return a / np.linalg.norm(a, axis=1)[:, None]
+
+
+def expected_table(slip_set):
+    key, ns = {1: ("fcc", 12), 8: ("bcc48", 48), 80: ("bcc48", 24)}[slip_set]
+    return unit_rows(TABLES[key]["b"][:ns]), unit_rows(TABLES[key]["n"][:ns])
+
+
+def oracle_table(c):
+    ns = c.nslip
+    return (np.array(c.bi[:]).reshape(-1, 3)[:ns], np.array(c.ni[:]).reshape(-1, 3)[:ns])
+
+
+def host_lib():
+    import test_host_voxel as H
+    L = H.host_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.cpfft_host_slip_systems.argtypes = [ctypes.c_int, dp, dp]
+    L.cpfft_host_orientation.argtypes = [ctypes.c_void_p, dp, dp, dp, dp]
+    return L
+
+
+def library_table(slip_set):
+    L = host_lib()
+    b, n = np.zeros(48 * 3), np.zeros(48 * 3)
+    ns = L.cpfft_host_slip_systems(slip_set, O.P(b), O.P(n))
+    return b.reshape(48, 3)[:ns], n.reshape(48, 3)[:ns]
+
+
+# ---------------------------------------------------------------- slip tables
+@pytest.mark.parametrize("slip_set", [1, 8, 80])
+def test_slip_tables_match_the_reference_tables(slip_set):
+    B, N = expected_table(slip_set)
+    ob, on = oracle_table(O.crystal(slip_set=slip_set))
+    lb, ln = library_table(slip_set)
+    for b, n in ((ob, on), (lb, ln)):
+        assert b.shape == B.shape
+        assert abs(b - B).max() < 1e-15 and abs(n - N).max() < 1e-15
+
+
+def _family(v):
+    """sorted absolute Miller indices of an integer triple"""
+    return tuple(sorted(abs(int(x)) for x in v))
+
+
+@pytest.mark.parametrize("key,families", [
+    ("fcc", [((1, 1, 1), (0, 1, 1), 12)]),
+    ("bcc48", [((0, 1, 1), (1, 1, 1), 12), ((1, 1, 2), (1, 1, 1), 12),
+               ((1, 2, 3), (1, 1, 1), 24)])])
+def test_slip_tables_are_complete_families(key, families):
+    """Independent of the reference text: every system lies in its plane,
+    and each block of the table is the complete family {n}<b> of cubic
+    crystallography, each system once (a system and its negative are one)."""
+    bi = np.array(TABLES[key]["b"])
+    ni = np.array(TABLES[key]["n"])
+    b, n = unit_rows(bi), unit_rows(ni)
+    assert np.allclose(np.linalg.norm(b, axis=1), 1) and np.allclose(np.linalg.norm(n, axis=1), 1)
+    assert abs((b * n).sum(axis=1)).max() < 1e-15
+    start = 0
+    for nfam, bfam, count in families:
+        blk = slice(start, start + count)
+        assert all(_family(v) == nfam for v in ni[blk])
+        assert all(_family(v) == bfam for v in bi[blk])
+        # all systems of the family: plane normals up to sign, directions
+        # in the plane up to sign, from the permutations and signs of the
+        # family's indices
+        import itertools
+        planes = {tuple(p) for p in itertools.permutations(nfam)}
+        normals = set()
+        for p in planes:
+            for sg in itertools.product((1, -1), repeat=3):
+                v = tuple(s * x for s, x in zip(sg, p))
+                if v > tuple(-x for x in v):
+                    normals.add(v)
+        dirs = set()
+        for p in set(itertools.permutations(bfam)):
+            for sg in itertools.product((1, -1), repeat=3):
+                v = tuple(s * x for s, x in zip(sg, p))
+                if v > tuple(-x for x in v):
+                    dirs.add(v)
+        systems = {(nv, dv) for nv in normals for dv in dirs if np.dot(nv, dv) == 0}
+        got = set()
+        for nv, dv in zip(ni[blk], bi[blk]):
+            nv, dv = tuple(int(x) for x in nv), tuple(int(x) for x in dv)
+            nv = max(nv, tuple(-x for x in nv))
+            dv = max(dv, tuple(-x for x in dv))
+            got.add((nv, dv))
+        assert len(got) == count == len(systems) and got == systems
+        start += count
+
+
+# ------------------------------------------------------------------ stiffness
+def _voigt_to_c4(Cv):
+    return np.array([[[[Cv[VI[a][b], VI[c][d]] for d in range(3)] for c in range(3)]
+                      for b in range(3)] for a in range(3)])
+
+
+def _c4_to_voigt(C4):
+    idx = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]
+    return np.array([[C4[i + j] for j in idx] for i in idx])
+
+
+def test_cubic_stiffness_recovers_C11_C12_C44():
+    c = O.crystal()  # e, nu from C11 = 168400, C12 = 121400; mu = C44
+    S = np.array(c.stiff[:]).reshape(6, 6)
+    assert abs(S[0, 0] - 168400.0) < 1e-8 * 168400 and abs(S[1, 1] - S[0, 0]) < 1e-9
+    assert abs(S[0, 1] - 121400.0) < 1e-8 * 121400 and abs(S[1, 2] - S[0, 1]) < 1e-9
+    assert abs(S[3, 3] - 75400.0) < 1e-8 * 75400 and S[4, 4] == S[3, 3] == S[5, 5]
+    assert abs(S[0:3, 3:]).max() == 0 and abs(S[3:, 3:] - np.diag(np.diag(S)[3:])).max() == 0
+
+
+def test_isotropic_stiffness_is_lame():
+    e, nu = 200000.0, 0.3
+    c = O.crystal(e=e, nu=nu, mu=e / (2 * (1 + nu)))
+    S = np.array(c.stiff[:]).reshape(6, 6)
+    lam = e * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = e / (2 * (1 + nu))
+    want = np.zeros((6, 6))
+    want[:3, :3] = lam
+    want[range(3), range(3)] += 2 * mu
+    want[range(3, 6), range(3, 6)] = mu
+    assert abs(S - want).max() < 1e-9 * abs(want).max()
+
+
+@pytest.mark.parametrize("slip_set", [1, 80, 8])
+def test_library_orientation_invariants_are_tensor_rotations(slip_set):
+    """Rotated stiffness C'_ijkl = t_ia t_jb t_kc t_ld C_abcd with t = g^T
+    (g the Kocks matrix), and the Schmid tensor sym(b_s (x) n_s) and spin
+    skew(b_s (x) n_s) of the rotated b_s = t b, n_s = t n, in the Voigt and
+    skew-vector conventions of mm10_et2ev / mm10_wt2wv (mm10_a.f:1492-1534)."""
+    import cpfft_amd as C
+    L = host_lib()
+    mat = C.Material(slip_set=slip_set)
+    cm = C.CMaterial(**{k: getattr(mat, k) for k, _ in C.CMaterial._fields_})
+    ang = np.array([37.0, 61.0, -115.0])
+    Cl, ms, qs = np.zeros(36), np.zeros(48 * 6), np.zeros(48 * 3)
+    ns = L.cpfft_host_orientation(ctypes.byref(cm), O.P(ang), O.P(Cl), O.P(ms), O.P(qs))
+    t = kocks_matrix(ang).T
+    Cc = np.array(O.crystal(slip_set=slip_set).stiff[:]).reshape(6, 6)
+    Cr = _c4_to_voigt(np.einsum("ia,jb,kc,ld,abcd->ijkl", t, t, t, t, _voigt_to_c4(Cc)))
+    assert abs(Cl.reshape(6, 6) - Cr).max() < 1e-10 * abs(Cr).max()
+    B, N = expected_table(slip_set)
+    for s in range(ns):
+        A = np.outer(t @ B[s], t @ N[s])
+        Sy, Sk = 0.5 * (A + A.T), 0.5 * (A - A.T)
+        want_m = [Sy[0, 0], Sy[1, 1], Sy[2, 2], 2 * Sy[0, 1], 2 * Sy[1, 2], 2 * Sy[0, 2]]
+        want_q = [Sk[1, 2], Sk[0, 2], Sk[0, 1]]
+        assert np.allclose(ms[6 * s:6 * s + 6], want_m, atol=1e-15)
+        assert np.allclose(qs[3 * s:3 * s + 3], want_q, atol=1e-15)
+
+
+# -------------------------------------------------- flow rule and residual R1
+def _polar_R(F):
+    U, _, Vt = np.linalg.svd(F)
+    return U @ Vt
+
+
+def _voigt_eng(E):
+    return np.array([E[0, 0], E[1, 1], E[2, 2], 2 * E[0, 1], 2 * E[1, 2], 2 * E[0, 2]])
+
+
+def _stress_tensor(v):
+    return np.array([[v[0], v[3], v[5]], [v[3], v[1], v[4]], [v[5], v[4], v[2]]])
+
+
+def _voigt_stress(S):
+    return np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[1, 2], S[0, 2]])
+
+
+def _skew_vec(W):
+    return np.array([W[1, 2], W[0, 2], W[0, 1]])
+
+
+def _skew_mat(w):
+    M = np.zeros((3, 3))
+    M[1, 2], M[0, 2], M[0, 1] = w
+    return M - M.T
+
+
+def _symsw(s, w):
+    """mm10_symsw (mm10_b.f:1505-1533) as the reference writes it.  It is
+    not sym(S W) of the skew tensor of w: components 11 and 12 carry the
+    opposite sign on one term each (checked below), a quirk the residual
+    keeps."""
+    return np.array([
+        s[3] * w[2] - s[5] * w[1],
+        s[3] * w[2] - s[4] * w[0],
+        s[5] * w[1] + s[4] * w[0],
+        0.5 * (w[2] * (s[0] - s[1]) + w[0] * s[5] - w[1] * s[4]),
+        0.5 * (w[0] * (s[1] - s[2]) + w[1] * s[3] + w[2] * s[5]),
+        0.5 * (w[1] * (s[0] - s[2]) + w[0] * s[3] - w[2] * s[4])])
+
+
+def test_symsw_quirk():
+    rng = np.random.default_rng(5)
+    s, w = rng.standard_normal(6), rng.standard_normal(3)
+    M = _stress_tensor(s) @ _skew_mat(w)
+    true = _voigt_stress(0.5 * (M + M.T))
+    got = _symsw(s, w)
+    d = got - true
+    assert np.allclose(d[[1, 2, 4, 5]], 0, atol=1e-15)
+    assert np.isclose(d[0], 2 * s[3] * w[2]) and np.isclose(d[3], w[0] * s[5])
+
+
+def _numpy_R1(c, slip_set, ang, dt, Fn, Fn1, sn, snp1):
+    """R1 = sigma - sigma_n - C (D - sum dgamma_s ms_s) + 2 symsw(sigma,
+    sum dgamma_s qc_s) at the stored state n+1, with every ingredient
+    formed here from the inputs: D from the polar decomposition of
+    F_{n+1/2}, the stiffness and Schmid tensors rotated as tensors, the
+    Schmid tensors then rotated by Rp_n^T as mm10_setup does (mm10_rt2rve is
+    the stress-Voigt rotation and is applied to the engineering-shear
+    vector ms), qc the spin vectors rotated by R_{n+1} Rp_n^T."""
+    ns = c.nslip
+    t = kocks_matrix(ang).T
+    B, N = expected_table(slip_set)
+    Cc = np.array(c.stiff[:]).reshape(6, 6)
+    Crot = _c4_to_voigt(np.einsum("ia,jb,kc,ld,abcd->ijkl", t, t, t, t, _voigt_to_c4(Cc)))
+    Fh = 0.5 * (Fn + Fn1)
+    Rh = _polar_R(Fh)
+    L = (Fn1 - Fn) @ np.linalg.inv(Fh)
+    D = _voigt_eng(Rh.T @ (0.5 * (L + L.T)) @ Rh)
+    Rn1 = _polar_R(Fn1)
+    Rp_n = sn[6:15].reshape(3, 3).T
+    sig = snp1[0:6]
+    sig_n = sn[0:6]
+    tt = snp1[15]
+    dg = c.eps_dot_0_y * dt
+    ms, qc = [], []
+    for s in range(ns):
+        A = np.outer(t @ B[s], t @ N[s])
+        m0 = _voigt_eng(0.5 * (A + A.T))
+        ms.append(_voigt_stress(Rp_n.T @ _stress_tensor(m0) @ Rp_n))
+        Q = Rn1 @ Rp_n.T
+        qc.append(_skew_vec(Q @ (0.5 * (A - A.T)) @ Q.T))
+    ms, qc = np.array(ms), np.array(qc)
+    tau = ms @ sig
+    dgam = dg / tt * np.abs(tau / tt) ** (c.rate_n - 1) * tau
+    stored = snp1[23:23 + ns]
+    # R1 with the stored D (the polar noise of D would otherwise dominate;
+    # D itself is compared with the one formed here by the caller)
+    R1 = sig - sig_n - Crot @ (snp1[17:23] - dgam @ ms) + 2 * _symsw(sig, dgam @ qc)
+    return R1, dgam, stored, D
+
+
+@pytest.mark.parametrize("slip_set", [1, 80, 8])
+def test_flow_rule_and_R1_vanish_at_the_converged_state(slip_set):
+    c = O.crystal(slip_set=slip_set, jac_exact=1)
+    c.rtol = c.atol = c.atol1 = c.rtol1 = 1e-13
+    c.miter = 60
+    ang = np.array([23.0, 48.0, 71.0])
+    dt = 0.25
+    s0 = O.state_init(c, 1)[:, 0]
+    F1 = np.diag([1 + 1.2e-3, 1 - 6e-4, 1 - 6e-4]) + 2e-4 * np.array(
+        [[0, 1, -0.5], [0.3, 0, 0.8], [-0.2, 0.6, 0]])
+    F2 = F1 + np.diag([8e-4, -4e-4, -4e-4]) + 1e-4 * np.array(
+        [[0, -0.4, 0.7], [0.5, 0, -0.3], [0.9, 0.1, 0]])
+    # step 1 from the initial state, then step 2 from its stored state
+    for step, Fn, Fn1, sn in ((1, I3, F1, s0), (2, F1, F2, None)):
+        if sn is None:
+            sn = s1
+        P, K, s1, st, fail = O.voxel_update(c, ang, step, 1, dt, Fn, Fn1, sn)
+        assert fail == 0 and st[1] > 0  # plastic Newton iterations
+        R1, dgam, stored, D = _numpy_R1(c, slip_set, ang, dt, Fn, Fn1, sn, s1)
+        sig = s1[0:6]
+        # the reference's Cardano polar decomposition (polar.f) against the
+        # SVD: equal to its own rounding noise (a few 1e-10 here)
+        assert abs(s1[17:23] - D).max() <= 1e-8 * abs(D).max(), step
+        assert abs(dgam).max() > 1e-6, step  # plastic flow
+        assert abs(dgam - stored).max() <= 1e-10 * abs(stored).max(), step
+        assert abs(R1).max() <= 1e-9 * abs(sig).max(), (step, abs(R1).max())
+        if step == 2:
+            assert abs(sn[6:15].reshape(3, 3) - I3).max() > 1e-6  # Rp_n != I
+
+
+def test_R1_check_detects_plausible_mistakes():
+    """The same numpy R1 with a flipped spin term, or Schmid tensors not
+    rotated by Rp_n, does not vanish at the converged state: the check above
+    is sensitive to the mistakes it is meant to catch."""
+    c = O.crystal(jac_exact=1)
+    c.rtol = c.atol = c.atol1 = c.rtol1 = 1e-13
+    c.miter = 60
+    ang = np.array([23.0, 48.0, 71.0])
+    dt = 0.25
+    s0 = O.state_init(c, 1)[:, 0]
+    F1 = np.diag([1 + 1.2e-3, 1 - 6e-4, 1 - 6e-4]) + 2e-4 * np.array(
+        [[0, 1, -0.5], [0.3, 0, 0.8], [-0.2, 0.6, 0]])
+    F2 = F1 + np.diag([8e-4, -4e-4, -4e-4]) + 1e-4 * np.array(
+        [[0, -0.4, 0.7], [0.5, 0, -0.3], [0.9, 0.1, 0]])
+    _, _, s1, _, f1 = O.voxel_update(c, ang, 1, 1, dt, I3, F1, s0)
+    _, _, s2, _, f2 = O.voxel_update(c, ang, 2, 1, dt, F1, F2, s1)
+    assert f1 == f2 == 0
+    R1, _, _, _ = _numpy_R1(c, 1, ang, dt, F1, F2, s1, s2)
+    sig = s2[0:6]
+    assert abs(R1).max() <= 1e-9 * abs(sig).max()
+    # flipped spin sign: R1 - 2 symsw(...) - 2 symsw(...)
+    ms_wrong = s1.copy()
+    ms_wrong[6:15] = np.eye(3).ravel()  # pretend Rp_n = I
+    R1w, _, _, _ = _numpy_R1(c, 1, ang, dt, F1, F2, ms_wrong, s2)
+    assert abs(R1w).max() > 1e3 * abs(R1).max() + 1e-9
+    global _symsw
+    keep = _symsw
+    try:
+        _symsw = lambda s, w: -keep(s, w)  # noqa: E731
+        R1s, _, _, _ = _numpy_R1(c, 1, ang, dt, F1, F2, s1, s2)
+    finally:
+        _symsw = keep
+    assert abs(R1s).max() > 1e3 * abs(R1).max() + 1e-9
+
+
+# ----------------------------------------------------------- load-step loop
+def _single_voxel_mixed_bc(c, ang, dt, BC, isNBC):
+    """Per load step: F_xx (and every other prescribed component) from BC,
+    P_yy = P_zz = 0 solved by Newton on (F_yy, F_zz) with a finite-difference
+    Jacobian of the single-voxel update; then the state rolls over."""
+    nstep = BC.shape[0]
+    s = O.state_init(c, 1)[:, 0]
+    Fn = I3.copy()
+    free = [k for k in range(9) if isNBC[k]]
+    out = np.zeros((nstep, 9))
+    F = Fn.copy()
+    for k in range(nstep):
+        Fv = F.ravel().copy()
+        for j in range(9):
+            if not isNBC[j]:
+                Fv[j] = BC[k, j]
+
+        def Pof(x):
+            G = Fv.copy()
+            G[free] = x
+            return O.voxel_update(c, ang, k + 1, 1, dt, Fn, G.reshape(3, 3), s)
+
+        x = Fv[free].copy()
+        for _ in range(30):
+            r = Pof(x)[0][free] - BC[k, free]
+            if abs(r).max() < 1e-11:
+                break
+            J = np.zeros((len(free), len(free)))
+            for m in range(len(free)):
+                h = 1e-8
+                xp = x.copy()
+                xp[m] += h
+                J[:, m] = (Pof(xp)[0][free] - BC[k, free] - r) / h
+            x = x - np.linalg.solve(J, r)
+        P, _, s1, _, fail = Pof(x)
+        assert fail == 0
+        out[k] = P
+        Fv[free] = x
+        F = Fv.reshape(3, 3)
+        Fn, s = F.copy(), s1
+    return out
+
+
+@pytest.mark.parametrize("N,gamma_mode", [(5, 0), (4, 1)])
+def test_fft_nr3_homogeneous_single_crystal_equals_one_voxel(N, gamma_mode):
+    c = O.crystal(jac_exact=1)
+    ang1 = np.array([15.0, 35.0, 55.0])
+    ang = np.repeat(ang1[:, None], N ** 3, axis=1)
+    nstep, dt = 6, 0.25
+    BC = np.zeros((nstep, 9))
+    BC[:, 0] = 1 + 2.5e-4 * np.arange(1, nstep + 1)  # F_xx, rate 1e-3/s
+    BC[:, 8] = BC[:, 4] = 0.0  # P_yy = P_zz = 0
+    isNBC = np.zeros(9, np.int32)
+    isNBC[[4, 8]] = 1
+    Pbar, cnt, rc, _ = O.fft_nr3(c, N, ang, dt, BC, isNBC, 1e-10, 1e-8, 25, gamma_mode)
+    assert rc == 0
+    ref = _single_voxel_mixed_bc(c, ang1, dt, BC, isNBC)
+    # the mixed-BC loop stops at |P_bar - P_BC| <= tolNR |P_bar| (1e-8)
+    assert abs(Pbar - ref).max() <= 1e-7 * abs(ref).max(), abs(Pbar - ref).max()
+    assert abs(ref[:, 0]).max() > 20  # past yield (tau_y = 16 MPa)
+    assert np.all(np.diff(Pbar[:, 0]) > 0)
