@@ -108,12 +108,49 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
         "unit": "load steps/s",
         "cores": threads,
         "kind": "port",
+        "implementation": ("oracle/ C restatement with OpenMP: the reference's complex-to-complex "
+                           "transform with phase factors, radix-2 for power-of-two N (not MKL), "
+                           "the mm10 update voxel by voxel"),
         "sample": (f"oracle G_K_dF with K4 once at {wl.N}^3 ({t_gkdf:.2f} s) and oracle "
                    f"drive_eps_sig on {nv} voxels of the current {wl.N}^3 state "
                    f"({t_des * nv / V:.2f} s, x{V / nv:.0f} for all voxels); one load step = "
                    f"{n_gkdf:.0f} G_K_dF + {n_des:.1f} drive_eps_sig calls (this run's counts)"),
         "seconds_per_step": step_s,
     }
+
+
+def validate_cpu_baseline(name, nstep):
+    """The extrapolated CPU baseline against a whole oracle run: the oracle
+    runs load steps 1..nstep of the workload (the initial tangent included)
+    and is timed end to end, and cpu_baseline's estimate for the same work
+    is formed from the same operator counts (the GPU run of those steps,
+    identical algorithm)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    wl = WORKLOADS[name]
+    BC, isNBC = wl.bc()
+    ang = wl.angles()
+    s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    rc, _, cnt, _ = s.compute(BC[:nstep], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    if rc != 0:
+        raise SystemExit(f"GPU run failed with code {rc}")
+    k = s.counters()  # since creation: G_K_dF applications, drive_eps_sig calls
+    est = cpu_baseline(wl, s, (k[0], k[2]))
+    s.close()
+    print(f"[validate] estimate {est['seconds_per_step']:.1f} s for steps 1-{nstep}; "
+          "running the oracle", flush=True)
+    oc = O.crystal(slip_set=wl.material.slip_set, jac_exact=wl.material.jac_exact)
+    t = time.perf_counter()
+    Po, co, rco, _ = O.fft_nr3(oc, wl.N, ang, wl.dt, BC[:nstep], isNBC, wl.tolPCG, wl.tolNR,
+                               wl.maxIter, gamma_mode=wl.gamma_mode)
+    measured = time.perf_counter() - t
+    return {"workload": f"{name}, load steps 1-{nstep} (initial tangent included)",
+            "cores": est["cores"], "oracle_rc": rco,
+            "measured_s": measured, "extrapolated_s": est["seconds_per_step"],
+            "measured_over_extrapolated": measured / est["seconds_per_step"],
+            "G_K_dF_applications": k[0], "drive_eps_sig_calls": k[2],
+            "gpu_counters_per_step": cnt.tolist(), "oracle_counters_per_step": co.tolist(),
+            "sample": est["sample"]}
 
 
 def workload_text(wl):
@@ -278,11 +315,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--threads", type=int, default=8, help="host threads of a batched workload")
+    ap.add_argument("--validate-cpu-baseline", type=int, default=0, metavar="NSTEP",
+                    help="time a whole oracle run of NSTEP load steps of --workload against "
+                         "the extrapolated CPU baseline, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the solver step is built and measured on one GPU's share)", file=sys.stderr)
         sys.exit(2)
+    if args.validate_cpu_baseline > 0:
+        print(json.dumps(validate_cpu_baseline(args.workload, args.validate_cpu_baseline)),
+              flush=True)
+        return
     if args.workload in BATCH_WORKLOADS:
         print(json.dumps(batch_bench(args.workload, args.warmup, args.steps, args.threads)),
               flush=True)
@@ -358,6 +402,16 @@ def main():
         },
         "Pbar_xx_last": float(Pbar[-1, 0]),
     }
+    # strain covered per second, and the rate in the benchmark config's own
+    # unit of 1e-3 strain per load step (its load steps are 4x those here)
+    dstrain = float(BC_all[1, 0] - BC_all[0, 0]) if wl.cycles == 0 else wl.strain * 4 * wl.cycles / wl.nstep
+    out["strain_per_step"] = dstrain
+    out["strain_per_s"] = steps_per_s * dstrain
+    out["config_step_equivalent"] = {
+        "value": steps_per_s * dstrain / 1e-3, "unit": "load steps of 1e-3 strain per s",
+        "note": ("the configs' 1e-3 strain per step does not converge with the reference "
+                 "algorithm (tests/test_gpu_step_size.py); this run covers the same strain "
+                 "in steps of {:g}".format(dstrain))}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, s, (n_gkdf / K, n_des / K))
     s.close()

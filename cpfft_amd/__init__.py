@@ -18,6 +18,7 @@ LIB = os.path.join(PKG, "libcpfft_amd.so")
 DRIVER = os.path.join(PKG, "libcpfft_driver.so")
 
 FN, FN1, PN, PN1, DFM, B, K4, STATE_N, STATE_NP1, WORK = range(10)
+WARM_NETWORK, WARM_PREVIOUS = 1, 2  # include/cpfft.h CPFFT_WARM_*
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -39,7 +40,7 @@ class CMaterial(ctypes.Structure):
 # every symbol include/cpfft.h declares
 EXPORTS = [
     "cpfft_create", "cpfft_destroy", "cpfft_last_error", "cpfft_nslip", "cpfft_field",
-    "cpfft_upload", "cpfft_download", "cpfft_reset", "cpfft_sync", "cpfft_driver_state",
+    "cpfft_upload", "cpfft_download", "cpfft_download_voxels", "cpfft_reset", "cpfft_sync", "cpfft_driver_state",
     "cpfft_gkdf", "cpfft_pcg",
     "cpfft_drive_eps_sig", "cpfft_tangent_homo", "cpfft_update", "cpfft_dnrm2", "cpfft_daxpy",
     "cpfft_dscal", "cpfft_dcopy", "cpfft_fill9", "cpfft_mean9", "cpfft_f2disp",
@@ -47,7 +48,9 @@ EXPORTS = [
     "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_set_cg_maxit", "cpfft_autotune",
     "cpfft_get_tune", "cpfft_set_tune",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
-    "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_stats",
+    "cpfft_set_warm_start",
+    "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_predict",
+    "cpfft_mlp_stats",
 ]
 
 # every symbol include/cpfft_driver.h declares (libcpfft_driver.so)
@@ -67,6 +70,14 @@ class CBatchItem(ctypes.Structure):
 
 MLP_IN, MLP_HID, MLP_OUT = 42, 128, 12
 MLP_NPARAM, MLP_NNORM = 23564, 108
+
+
+def mlp_sizes(nslip):
+    """(nin, nout, nparam, nnorm) of the warm-start network for a crystal of
+    nslip slip systems (include/cpfft.h CPFFT_MLP_NIN/NOUT/NPARAMS/NNORMS)"""
+    nin = 18 + 2 * nslip
+    h = MLP_HID
+    return nin, nslip, h * (nin + 1) + h * (h + 1) + nslip * (h + 1), 2 * nin + 2 * nslip
 
 _lib = None
 _drv = None
@@ -97,6 +108,7 @@ def lib():
         L.cpfft_driver_state.restype = _dp
         L.cpfft_upload.argtypes = [_vp, ctypes.c_int, _dp]
         L.cpfft_download.argtypes = [_vp, ctypes.c_int, _dp]
+        L.cpfft_download_voxels.argtypes = [_vp, ctypes.c_int, ctypes.c_long, ctypes.c_long, _dp]
         L.cpfft_reset.argtypes = [_vp]
         L.cpfft_sync.argtypes = [_vp]
         L.cpfft_gkdf.argtypes = [_vp, _vp, _vp, ctypes.c_int]
@@ -119,9 +131,11 @@ def lib():
         L.cpfft_mlp_log.argtypes = [_vp, ctypes.c_int, ctypes.c_long]
         L.cpfft_mlp_train.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
         L.cpfft_mlp_warm_start.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_set_warm_start.argtypes = [_vp, ctypes.c_int]
         L.cpfft_mlp_params.argtypes = [_vp, _fp, ctypes.c_int]
         L.cpfft_mlp_forward.argtypes = [_vp, _fp, ctypes.c_int, _fp]
         L.cpfft_mlp_grad.argtypes = [_vp, _fp, _fp, ctypes.c_int, _fp, _dp]
+        L.cpfft_mlp_predict.argtypes = [_vp, ctypes.c_int, _fp, _fp]
         L.cpfft_mlp_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong)]
         _lib = L
     return _lib
@@ -205,6 +219,14 @@ class Solver:
         _check(lib().cpfft_download(self.h, f, _p(out)), "cpfft_download")
         rows = self._len(f) // self.V
         return out.reshape(rows, self.V)
+
+    def download_voxels(self, f, e0, n):
+        """voxels [e0, e0+n) of every component of field f, [rows][n]"""
+        rows = self._len(f) // self.V
+        out = np.empty(rows * n)
+        _check(lib().cpfft_download_voxels(self.h, f, int(e0), int(n), _p(out)),
+               "cpfft_download_voxels")
+        return out.reshape(rows, n)
 
     def reset(self):
         _check(lib().cpfft_reset(self.h), "cpfft_reset")
@@ -327,30 +349,46 @@ class Solver:
     def mlp_warm_start(self, on):
         _check(lib().cpfft_mlp_warm_start(self.h, int(on)), "cpfft_mlp_warm_start")
 
+    def set_warm_start(self, mode):
+        """Warm-start sources of the local solves: WARM_NETWORK | WARM_PREVIOUS
+        (0: the reference's start)."""
+        _check(lib().cpfft_set_warm_start(self.h, int(mode)), "cpfft_set_warm_start")
+
     def mlp_params(self, new=None):
         """Get (new=None) or set the flat parameters + normalisation vector."""
         if new is None:
-            out = np.zeros(MLP_NPARAM + MLP_NNORM, dtype=np.float32)
+            _, _, npar, nnorm = mlp_sizes(self.nslip)
+            out = np.zeros(npar + nnorm, dtype=np.float32)
             _check(lib().cpfft_mlp_params(self.h, _pf(out), 1), "cpfft_mlp_params")
             return out
         buf = np.ascontiguousarray(new, dtype=np.float32)
-        assert buf.size == MLP_NPARAM + MLP_NNORM
+        _, _, npar, nnorm = mlp_sizes(self.nslip)
+        assert buf.size == npar + nnorm
         _check(lib().cpfft_mlp_params(self.h, _pf(buf), 0), "cpfft_mlp_params")
 
     def mlp_forward(self, Xn):
         X = np.ascontiguousarray(Xn, dtype=np.float32)
-        Y = np.zeros((X.shape[0], MLP_OUT), dtype=np.float32)
+        Y = np.zeros((X.shape[0], mlp_sizes(self.nslip)[1]), dtype=np.float32)
         _check(lib().cpfft_mlp_forward(self.h, _pf(X), X.shape[0], _pf(Y)), "cpfft_mlp_forward")
         return Y
 
     def mlp_grad(self, Xn, Tn):
         X = np.ascontiguousarray(Xn, dtype=np.float32)
         T = np.ascontiguousarray(Tn, dtype=np.float32)
-        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        g = np.zeros(mlp_sizes(self.nslip)[2], dtype=np.float32)
         loss = ctypes.c_double(0.0)
         _check(lib().cpfft_mlp_grad(self.h, _pf(X), _pf(T), X.shape[0], _pf(g), ctypes.byref(loss)),
                "cpfft_mlp_grad")
         return g, loss.value
+
+    def mlp_predict(self, step):
+        """(raw network inputs [V][nin], predicted slip increments as
+        normalised resolved shears [nout][V]) for the current state"""
+        nin, nout, _, _ = mlp_sizes(self.nslip)
+        X = np.zeros((self.V, nin), dtype=np.float32)
+        ws = np.zeros((nout, self.V), dtype=np.float32)
+        _check(lib().cpfft_mlp_predict(self.h, int(step), _pf(X), _pf(ws)), "cpfft_mlp_predict")
+        return X, ws
 
     def mlp_stats(self):
         out = (ctypes.c_longlong * 4)()

@@ -75,7 +75,7 @@ struct cpfft_ctx {
   // learned warm start
   MlpBuffers mlp;
   bool mlp_ready = false;  // buffers allocated
-  int mlp_on = 0;
+  int warm = 0;  // warm-start sources: CPFFT_WARM_NETWORK | CPFFT_WARM_PREVIOUS
   long mlp_count = 0;      // pairs held in the log (at most its capacity)
   long mlp_total = 0;      // pairs ever logged; the log is a ring over the last ones
   long mlp_cap = 0;        // ring size (the capacity of the last cpfft_mlp_init)
@@ -155,7 +155,7 @@ extern "C" int cpfft_destroy(cpfft_ctx* c) {
     MlpBuffers& b = c->mlp;
     void* mb[] = {b.params, b.pbf, b.grads, b.m, b.v, b.norm, b.logX, b.logT, b.Xb, b.Xf, b.Tb,
                   b.H1, b.H2, b.Y, b.dY, b.dZ1, b.dZ2, b.H1b, b.H2b, b.dYb, b.dZ1b, b.dZ2b,
-                  b.part, b.ws};
+                  b.part, b.Xv, b.ws};
     for (void* q : mb)
       if (q) (void)hipFree(q);
   }
@@ -296,6 +296,16 @@ extern "C" int cpfft_upload(cpfft_ctx* c, int f, const double* h) {
 extern "C" int cpfft_download(cpfft_ctx* c, int f, double* h) {
   if (!c || f < 0 || f >= CPFFT_NFIELDS) return fail(-1, "cpfft_download: bad field");
   HIP_OK(hipMemcpyAsync(h, c->fields[f], c->field_len[f] * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int cpfft_download_voxels(cpfft_ctx* c, int f, long e0, long n, double* h) {
+  if (!c || f < 0 || f >= CPFFT_NFIELDS) return fail(-1, "cpfft_download_voxels: bad field");
+  if (e0 < 0 || n < 1 || e0 + n > c->V || !h) return fail(-1, "cpfft_download_voxels: bad range");
+  const long ncomp = c->field_len[f] / c->V;
+  HIP_OK(hipMemcpy2DAsync(h, n * sizeof(double), c->fields[f] + e0, c->V * sizeof(double),
+                          n * sizeof(double), ncomp, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -801,20 +811,17 @@ extern "C" int cpfft_pcg(cpfft_ctx* c, const double* b, double* x, double tol, i
   return 0;
 }
 
-// predicted slip increments of every voxel into c->mlp.ws (network inference
-// in chunks of the work-buffer size)
+// predicted slip increments of every voxel into c->mlp.ws: the normalised
+// inputs of all voxels, then one inference launch
+template <int NS>
 static void mlp_predict(cpfft_ctx* c, int step) {
   MlpBuffers& b = c->mlp;
   mlp_weights_to_bf16(b, c->stream);
-  for (long e0 = 0; e0 < c->V; e0 += b.rows) {
-    const long nr = std::min((long)b.rows, c->V - e0);
-    launch_mlp_features(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
-                        c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
-                        nullptr, e0, 1, nr, b.Xf, nullptr, c->stream);
-    mlp_norm_inputs(b, nr, b.Xf, b.Xb, c->stream);
-    mlp_forward(b, (int)nr, b.Xb, b.Y, c->stream);
-    launch_mlp_denorm(c->V, e0, nr, b.Y, b.norm, b.ws, c->stream);
-  }
+  launch_mlp_features_norm<NS>(c->V, c->mp, (const Orient<NS>*)c->orients, c->oid, step,
+                               c->fields[CPFFT_FN], c->fields[CPFFT_FN1],
+                               c->fields[CPFFT_STATE_N], b.norm, mlp_nin_pad(b.d.nin), b.Xv,
+                               c->stream);
+  mlp_infer(b, c->V, c->stream);
 }
 
 extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
@@ -824,26 +831,30 @@ extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
   HIP_OK(hipMemcpyAsync(before, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
   tstart(c, 1);
   const float* ws = nullptr;
-  if (c->mlp_on && iiter > 0 && c->nslip == 12) {
-    mlp_predict(c, step);
+  const int wp = (c->warm & CPFFT_WARM_PREVIOUS) && iiter > 0;
+  if ((c->warm & CPFFT_WARM_NETWORK) && iiter > 0 && c->nslip <= kMlpMaxOut) {
+    if (c->nslip == 12)
+      mlp_predict<12>(c, step);
+    else
+      mlp_predict<24>(c, step);
     ws = c->mlp.ws;
-    c->ws_updates += c->V;
   }
+  if (ws || wp) c->ws_updates += c->V;
   if (c->nslip == 12)
     launch_drive_eps_sig<12>(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
-                             c->fields[CPFFT_K4], ws, c->dcnt, c->stream);
+                             c->fields[CPFFT_K4], ws, wp, c->dcnt, c->stream);
   else if (c->nslip == 24)
     launch_drive_eps_sig<24>(c->V, c->mp, (const Orient<24>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
-                             c->fields[CPFFT_K4], nullptr, c->dcnt, c->stream);
+                             c->fields[CPFFT_K4], ws, wp, c->dcnt, c->stream);
   else
     launch_drive_eps_sig<48>(c->V, c->mp, (const Orient<48>*)c->orients, c->oid, step, iiter,
                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
                              c->fields[CPFFT_STATE_NP1], c->fields[CPFFT_PN1], c->tan6,
-                             c->fields[CPFFT_K4], nullptr, c->dcnt, c->stream);
+                             c->fields[CPFFT_K4], nullptr, wp, c->dcnt, c->stream);
   tstop(c, 1);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(after, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
@@ -1034,60 +1045,66 @@ extern "C" int cpfft_counters(cpfft_ctx* c, long long out[8]) {
 // ------------------------------------------------------ learned warm start
 extern "C" int cpfft_mlp_init(cpfft_ctx* c, uint64_t seed, long capacity, int batch) {
   if (!c) return fail(-1, "cpfft_mlp_init: null context");
-  if (c->nslip != 12) return fail(-1, "cpfft_mlp_init: the warm start is defined for fcc (12 slip systems)");
+  if (c->nslip > kMlpMaxOut)
+    return fail(-1, "cpfft_mlp_init: the warm start serves fcc (12) and bcc24 (24 slip systems)");
   if (capacity < 1 || batch < 16) return fail(-1, "cpfft_mlp_init: capacity >= 1 and batch >= 16 required");
   MlpBuffers& b = c->mlp;
   if (!c->mlp_ready) {
-    const int rows = std::max(batch, 65536);
+    const int rows = std::max(batch, 4096);
+    b.d = mlp_dims(c->nslip);
+    const MlpDims& d = b.d;
     b.rows = rows;
     b.cap = capacity;
-    HIP_OK(hipMalloc(&b.params, kMlpParams * sizeof(float)));
-    HIP_OK(hipMalloc(&b.pbf, kMlpParams * sizeof(uint16_t)));
-    HIP_OK(hipMalloc(&b.grads, kMlpParams * sizeof(float)));
-    HIP_OK(hipMalloc(&b.m, kMlpParams * sizeof(float)));
-    HIP_OK(hipMalloc(&b.v, kMlpParams * sizeof(float)));
-    HIP_OK(hipMalloc(&b.norm, kMlpNorm * sizeof(float)));
-    HIP_OK(hipMalloc(&b.logX, capacity * kMlpIn * sizeof(float)));
-    HIP_OK(hipMalloc(&b.logT, capacity * kMlpOut * sizeof(float)));
-    HIP_OK(hipMalloc(&b.Xb, (long)rows * kMlpIn * sizeof(uint16_t)));
-    HIP_OK(hipMalloc(&b.Xf, (long)rows * kMlpIn * sizeof(float)));
-    HIP_OK(hipMalloc(&b.Tb, (long)rows * kMlpOut * sizeof(float)));
+    HIP_OK(hipMalloc(&b.params, d.nparam * sizeof(float)));
+    HIP_OK(hipMalloc(&b.pbf, d.nparam * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.grads, d.nparam * sizeof(float)));
+    HIP_OK(hipMalloc(&b.m, d.nparam * sizeof(float)));
+    HIP_OK(hipMalloc(&b.v, d.nparam * sizeof(float)));
+    HIP_OK(hipMalloc(&b.norm, d.nnorm * sizeof(float)));
+    HIP_OK(hipMalloc(&b.logX, capacity * d.nin * sizeof(float)));
+    HIP_OK(hipMalloc(&b.logT, capacity * d.nout * sizeof(float)));
+    HIP_OK(hipMalloc(&b.Xb, (long)rows * d.nin * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.Xf, (long)rows * d.nin * sizeof(float)));
+    HIP_OK(hipMalloc(&b.Tb, (long)rows * d.nout * sizeof(float)));
     float** f32[] = {&b.H1, &b.H2, &b.dZ1, &b.dZ2};
     for (float** q : f32) HIP_OK(hipMalloc(q, (long)rows * kMlpHid * sizeof(float)));
-    HIP_OK(hipMalloc(&b.Y, (long)rows * kMlpOut * sizeof(float)));
-    HIP_OK(hipMalloc(&b.dY, (long)rows * kMlpOut * sizeof(float)));
+    HIP_OK(hipMalloc(&b.Y, (long)rows * d.nout * sizeof(float)));
+    HIP_OK(hipMalloc(&b.dY, (long)rows * d.nout * sizeof(float)));
     uint16_t** b16[] = {&b.H1b, &b.H2b, &b.dZ1b, &b.dZ2b};
     for (uint16_t** q : b16) HIP_OK(hipMalloc(q, (long)rows * kMlpHid * sizeof(uint16_t)));
-    HIP_OK(hipMalloc(&b.dYb, (long)rows * kMlpOut * sizeof(uint16_t)));
-    HIP_OK(hipMalloc(&b.part, (long)mlp_loss_parts(rows) * sizeof(double)));
-    HIP_OK(hipMalloc(&b.ws, (long)kMlpOut * c->V * sizeof(float)));
+    HIP_OK(hipMalloc(&b.dYb, (long)rows * d.nout * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.part, (long)mlp_loss_parts(b, rows) * sizeof(double)));
+    HIP_OK(hipMalloc(&b.Xv, c->V * mlp_nin_pad(d.nin) * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&b.ws, (long)d.nout * c->V * sizeof(float)));
     c->mlp_ready = true;
   } else if (batch > b.rows || capacity > b.cap) {
     return fail(-1, "cpfft_mlp_init: a larger batch or log than the first call allocated");
   }
+  const MlpDims& d = b.d;
   // Glorot-uniform weights, zero biases, identity normalisation
-  std::vector<float> h(kMlpParams, 0.0f);
+  std::vector<float> h(d.nparam, 0.0f);
   std::mt19937_64 rng(seed);
   auto fill = [&](long off, int fan_out, int fan_in) {
     const double a = sqrt(6.0 / (double)(fan_in + fan_out));
     std::uniform_real_distribution<double> U(-a, a);
     for (long i = 0; i < (long)fan_out * fan_in; ++i) h[off + i] = (float)U(rng);
   };
-  fill(kOffW1, kMlpHid, kMlpIn);
-  fill(kOffW2, kMlpHid, kMlpHid);
-  fill(kOffW3, kMlpOut, kMlpHid);
-  HIP_OK(hipMemcpy(b.params, h.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
-  HIP_OK(hipMemset(b.m, 0, kMlpParams * sizeof(float)));
-  HIP_OK(hipMemset(b.v, 0, kMlpParams * sizeof(float)));
-  std::vector<float> nrm(kMlpNorm, 0.0f);
-  for (int i = 0; i < kMlpIn; ++i) nrm[kMlpIn + i] = 1.0f;
-  for (int i = 0; i < kMlpOut; ++i) nrm[2 * kMlpIn + kMlpOut + i] = 1.0f;
-  HIP_OK(hipMemcpy(b.norm, nrm.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
+  fill(d.offW1, kMlpHid, d.nin);
+  fill(d.offW2, kMlpHid, kMlpHid);
+  fill(d.offW3, d.nout, kMlpHid);
+  HIP_OK(hipMemcpyAsync(b.params, h.data(), d.nparam * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  launch_fill_f32(d.nparam, 0.0f, b.m, c->stream);
+  launch_fill_f32(d.nparam, 0.0f, b.v, c->stream);
+  std::vector<float> nrm(d.nnorm, 0.0f);
+  for (int i = 0; i < d.nin; ++i) nrm[d.nin + i] = 1.0f;
+  for (int i = 0; i < d.nout; ++i) nrm[2 * d.nin + d.nout + i] = 1.0f;
+  HIP_OK(hipMemcpyAsync(b.norm, nrm.data(), d.nnorm * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   c->mlp_count = 0;
   c->mlp_total = 0;
   c->mlp_cap = capacity;
   c->mlp_steps = 0;
-  c->mlp_on = 0;
+  c->warm &= ~CPFFT_WARM_NETWORK;
   return 0;
 }
 
@@ -1102,10 +1119,16 @@ extern "C" int cpfft_mlp_log(cpfft_ctx* c, int step, long stride) {
   long pos = c->mlp_total % cap, done = 0;
   while (done < rows) {
     const long n = std::min(rows - done, cap - pos);
-    launch_mlp_features(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
-                        c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
-                        c->fields[CPFFT_STATE_NP1], done * stride, stride, n,
-                        b.logX + pos * kMlpIn, b.logT + pos * kMlpOut, c->stream);
+    if (c->nslip == 12)
+      launch_mlp_features<12>(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
+                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                              c->fields[CPFFT_STATE_NP1], done * stride, stride, n,
+                              b.logX + pos * b.d.nin, b.logT + pos * b.d.nout, c->stream);
+    else
+      launch_mlp_features<24>(c->V, c->mp, (const Orient<24>*)c->orients, c->oid, step,
+                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                              c->fields[CPFFT_STATE_NP1], done * stride, stride, n,
+                              b.logX + pos * b.d.nin, b.logT + pos * b.d.nout, c->stream);
     done += n;
     pos = (pos + n) % cap;
   }
@@ -1130,43 +1153,58 @@ extern "C" int cpfft_mlp_train(cpfft_ctx* c, int iters, double lr, uint64_t seed
   }
   HIP_OK(hipGetLastError());
   c->mlp_steps += iters;
-  const int np = mlp_loss_parts(rows);
+  const int np = mlp_loss_parts(b, rows);
   std::vector<double> hp(np);
   HIP_OK(hipMemcpyAsync(hp.data(), b.part, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   double acc = 0.0;
   for (double v : hp) acc += v;
-  if (loss) *loss = acc / ((double)rows * kMlpOut);
+  if (loss) *loss = acc / ((double)rows * b.d.nout);
+  return 0;
+}
+
+extern "C" int cpfft_set_warm_start(cpfft_ctx* c, int mode) {
+  if (!c) return fail(-1, "cpfft_set_warm_start: null context");
+  if (mode & ~(CPFFT_WARM_NETWORK | CPFFT_WARM_PREVIOUS))
+    return fail(-1, "cpfft_set_warm_start: unknown mode bits");
+  if ((mode & CPFFT_WARM_NETWORK) && !c->mlp_ready)
+    return fail(-1, "cpfft_set_warm_start: the network start needs cpfft_mlp_init first");
+  c->warm = mode;
   return 0;
 }
 
 extern "C" int cpfft_mlp_warm_start(cpfft_ctx* c, int on) {
   if (!c) return fail(-1, "cpfft_mlp_warm_start: null context");
-  if (on && !c->mlp_ready) return fail(-1, "cpfft_mlp_warm_start: cpfft_mlp_init first");
-  c->mlp_on = on ? 1 : 0;
-  return 0;
+  return cpfft_set_warm_start(c, on ? (c->warm | CPFFT_WARM_NETWORK) : (c->warm & ~CPFFT_WARM_NETWORK));
 }
 
 extern "C" int cpfft_mlp_params(cpfft_ctx* c, float* buf, int get) {
   if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_params: cpfft_mlp_init first");
   MlpBuffers& b = c->mlp;
+  const MlpDims& d = b.d;
   if (get) {
-    HIP_OK(hipMemcpy(buf, b.params, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(buf + kMlpParams, b.norm, kMlpNorm * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpyAsync(buf, b.params, d.nparam * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(buf + d.nparam, b.norm, d.nnorm * sizeof(float), hipMemcpyDeviceToHost,
+                          c->stream));
   } else {
-    HIP_OK(hipMemcpy(b.params, buf, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(b.norm, buf + kMlpParams, kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpyAsync(b.params, buf, d.nparam * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(b.norm, buf + d.nparam, d.nnorm * sizeof(float), hipMemcpyHostToDevice,
+                          c->stream));
   }
+  HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 static int mlp_upload_batch(cpfft_ctx* c, const float* Xn, const float* Tn, int n) {
   MlpBuffers& b = c->mlp;
   if (n < 1 || n > b.rows) return fail(-1, "cpfft_mlp: batch size out of range");
-  HIP_OK(hipMemcpy(b.Xf, Xn, (long)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice));
-  mlp_to_bf16((long)n * kMlpIn, b.Xf, b.Xb, c->stream);
-  if (Tn) HIP_OK(hipMemcpy(b.Tb, Tn, (long)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpyAsync(b.Xf, Xn, (long)n * b.d.nin * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  mlp_to_bf16((long)n * b.d.nin, b.Xf, b.Xb, c->stream);
+  if (Tn)
+    HIP_OK(hipMemcpyAsync(b.Tb, Tn, (long)n * b.d.nout * sizeof(float), hipMemcpyHostToDevice,
+                          c->stream));
   mlp_weights_to_bf16(b, c->stream);
+  HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -1177,7 +1215,9 @@ extern "C" int cpfft_mlp_forward(cpfft_ctx* c, const float* Xn, int n, float* Y)
   mlp_forward(c->mlp, n, c->mlp.Xb, c->mlp.Y, c->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->stream));
-  HIP_OK(hipMemcpy(Y, c->mlp.Y, (long)n * kMlpOut * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpyAsync(Y, c->mlp.Y, (long)n * c->mlp.d.nout * sizeof(float), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -1191,20 +1231,56 @@ extern "C" int cpfft_mlp_grad(cpfft_ctx* c, const float* Xn, const float* Tn, in
   mlp_backward(b, n, b.Xb, b.Tb, b.grads, b.part, c->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->stream));
-  HIP_OK(hipMemcpy(grad, b.grads, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
-  const int np = mlp_loss_parts(n);
+  const int np = mlp_loss_parts(b, n);
   std::vector<double> hp(np);
-  HIP_OK(hipMemcpy(hp.data(), b.part, np * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpyAsync(grad, b.grads, b.d.nparam * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(hp.data(), b.part, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   double acc = 0.0;
   for (double v : hp) acc += v;
-  if (loss) *loss = acc / ((double)n * kMlpOut);
+  if (loss) *loss = acc / ((double)n * b.d.nout);
+  return 0;
+}
+
+extern "C" int cpfft_mlp_predict(cpfft_ctx* c, int step, float* X, float* ws) {
+  if (!c || !c->mlp_ready) return fail(-1, "cpfft_mlp_predict: cpfft_mlp_init first");
+  MlpBuffers& b = c->mlp;
+  if (X) {
+    float* d = nullptr;
+    HIP_OK(hipMalloc(&d, c->V * b.d.nin * sizeof(float)));
+    if (c->nslip == 12)
+      launch_mlp_features<12>(c->V, c->mp, (const Orient<12>*)c->orients, c->oid, step,
+                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                              nullptr, 0, 1, c->V, d, nullptr, c->stream);
+    else
+      launch_mlp_features<24>(c->V, c->mp, (const Orient<24>*)c->orients, c->oid, step,
+                              c->fields[CPFFT_FN], c->fields[CPFFT_FN1], c->fields[CPFFT_STATE_N],
+                              nullptr, 0, 1, c->V, d, nullptr, c->stream);
+    const hipError_t e = hipMemcpyAsync(X, d, c->V * b.d.nin * sizeof(float),
+                                        hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    HIP_OK(e);
+    HIP_OK(e2);
+  }
+  if (ws) {
+    if (c->nslip == 12)
+      mlp_predict<12>(c, step);
+    else
+      mlp_predict<24>(c, step);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(ws, b.ws, c->V * b.d.nout * sizeof(float), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 
 extern "C" int cpfft_mlp_stats(cpfft_ctx* c, long long out[4]) {
   if (!c) return fail(-1, "cpfft_mlp_stats: null context");
-  long long d[8];
-  HIP_OK(hipMemcpy(d, c->dcnt, sizeof d, hipMemcpyDeviceToHost));
+  long long* d = c->hcnt;
+  HIP_OK(hipMemcpyAsync(d, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   out[0] = c->mlp_count;
   out[1] = c->mlp_steps;
   out[2] = c->ws_updates;

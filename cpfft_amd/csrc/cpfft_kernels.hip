@@ -475,6 +475,13 @@ void launch_fill9(long V, const double* v9, double* x, hipStream_t s) {
 void launch_fill(long n, double v, double* x, hipStream_t s) {
   k_fill<<<grid_for(n), kBlock, 0, s>>>(n, v, x);
 }
+__global__ void k_fill_f32(long n, float v, float* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+void launch_fill_f32(long n, float v, float* x, hipStream_t s) {
+  k_fill_f32<<<grid_for(n), kBlock, 0, s>>>(n, v, x);
+}
 void launch_state_init(long V, int W, double tt0, double* state, hipStream_t s) {
   k_state_init<<<grid_for((long)W * V), kBlock, 0, s>>>(V, W, tt0, state);
 }
@@ -715,18 +722,28 @@ __global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
                                                       double* __restrict__ Pn1,
                                                       double* __restrict__ tan6,
                                                       const float* __restrict__ ws,
+                                                      int warm_prev,
                                                       long long* __restrict__ counters) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   Counters cnt{0, 0, 0, 0, 0, 0};
   if (e < V) {
     const Orient<NS>& o = orients[oid[e]];
+    // this step's previous global iterate (stress, slip resistance), read
+    // before the update overwrites it
+    double xp[7];
+    if (warm_prev) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) xp[k] = snp1[k * V + e];
+      xp[6] = snp1[15 * V + e];
+    }
     voxel_update<NS>(
         P, o, step, iter,
         [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
         [&](int w) { return sn[w * V + e]; },
         [&](int w, double v) { snp1[w * V + e] = v; },
         [&](int c, double v) { Pn1[c * V + e] = v; },
-        [&](int c, double v) { tan6[c * V + e] = v; }, cnt, ws ? ws + e : nullptr, V);
+        [&](int c, double v) { tan6[c * V + e] = v; }, cnt, ws ? ws + e : nullptr, V,
+        warm_prev ? xp : nullptr);
   }
   // wavefront-level sums of the counters, one integer atomic per wavefront
   long long c[6] = {cnt.pred_it, cnt.upd_it, cnt.cuts, cnt.failed, cnt.ls_evals, cnt.ws_fallback};
@@ -757,21 +774,22 @@ template <int NS>
 void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
                           const int32_t* orient_id, int step, int iter, const double* Fn,
                           const double* Fn1, const double* sn, double* snp1, double* Pn1,
-                          double* tan6, double* K4, const float* ws, long long* counters,
-                          hipStream_t s) {
+                          double* tan6, double* K4, const float* ws, int warm_prev,
+                          long long* counters, hipStream_t s) {
   const int bs = 64;
   unsigned g = (unsigned)((V + bs - 1) / bs);
   k_drive_eps_sig<NS><<<g, bs, 0, s>>>(V, P, orients, orient_id, step, iter, Fn, Fn1, sn, snp1,
-                                       Pn1, tan6, ws, counters);
+                                       Pn1, tan6, ws, warm_prev, counters);
   k_cep2A<<<g, bs, 0, s>>>(V, Fn, Fn1, snp1, tan6, K4);
 }
 
 // ----------------------------------------------- warm-start features
 // rows r < nrows of voxels e = e0 + r * stride: raw network inputs into
-// X[r][42]; with T != nullptr also the converged slip increments of state
-// n+1, as u_s (slip_to_u), into T[r][12] (training pairs)
+// X[r][18 + 2 NS]; with T != nullptr also the converged slip increments of
+// state n+1, as u_s (slip_to_u), into T[r][NS] (training pairs)
+template <int NS>
 __global__ __launch_bounds__(64) void k_mlp_features(long V, MatParams P,
-                                                     const Orient<12>* __restrict__ orients,
+                                                     const Orient<NS>* __restrict__ orients,
                                                      const int32_t* __restrict__ oid, int step,
                                                      const double* __restrict__ Fn,
                                                      const double* __restrict__ Fn1,
@@ -779,55 +797,94 @@ __global__ __launch_bounds__(64) void k_mlp_features(long V, MatParams P,
                                                      const double* __restrict__ snp1, long e0,
                                                      long stride, long nrows,
                                                      float* __restrict__ X, float* __restrict__ T) {
+  constexpr int NIN = 18 + 2 * NS;
   const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   const long e = e0 + r * stride;
   if (e >= V) return;
-  float f[kMlpIn];
-  voxel_features(P, orients[oid[e]], step,
-                 [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
-                 [&](int w) { return sn[w * V + e]; }, f);
-  for (int k = 0; k < kMlpIn; ++k) X[r * kMlpIn + k] = f[k];
+  float f[NIN];
+  voxel_features<NS>(P, orients[oid[e]], step,
+                     [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
+                     [&](int w) { return sn[w * V + e]; }, f);
+  for (int k = 0; k < NIN; ++k) X[r * NIN + k] = f[k];
   if (T) {
     const double dg = P.alter_mode ? P.eps_dot_0_y * P.dt : 1.0;
-    for (int k = 0; k < kMlpOut; ++k) T[r * kMlpOut + k] = (float)slip_to_u(P, dg, snp1[(23 + k) * V + e]);
+    for (int k = 0; k < NS; ++k) T[r * NS + k] = (float)slip_to_u(P, dg, snp1[(23 + k) * V + e]);
   }
 }
 
-// predicted slip increments (as u_s) of rows r < nrows (voxels e0 + r)
-// from the normalised network output
-__global__ void k_mlp_denorm(long V, long e0, long nrows, const float* __restrict__ Y,
-                             const float* __restrict__ nrm, float* __restrict__ ws) {
-  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (r >= nrows || e0 + r >= V) return;
-  const float* mt = nrm + 2 * kMlpIn;
-  const float* st = mt + kMlpOut;
-  for (int k = 0; k < kMlpOut; ++k) ws[k * V + e0 + r] = Y[r * kMlpOut + k] * st[k] + mt[k];
+// every voxel's network inputs, normalised ((x - mean) / sd, nrm =
+// [mean | sd]) and rounded to BF16, into Xv[e][NINP] with the pad columns
+// zero: the A operand of the inference kernel
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (inputs are finite)
+  return (uint16_t)(u >> 16);
+}
+template <int NS>
+__global__ __launch_bounds__(64) void k_mlp_features_norm(long V, MatParams P,
+                                                          const Orient<NS>* __restrict__ orients,
+                                                          const int32_t* __restrict__ oid,
+                                                          int step, const double* __restrict__ Fn,
+                                                          const double* __restrict__ Fn1,
+                                                          const double* __restrict__ sn,
+                                                          const float* __restrict__ nrm, int ninp,
+                                                          uint16_t* __restrict__ Xv) {
+  constexpr int NIN = 18 + 2 * NS;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= V) return;
+  float f[NIN];
+  voxel_features<NS>(P, orients[oid[e]], step,
+                     [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
+                     [&](int w) { return sn[w * V + e]; }, f);
+  uint16_t* x = Xv + e * ninp;
+  for (int k = 0; k < ninp; k += 2) {
+    const uint16_t lo = k < NIN ? f32_to_bf16((f[k] - nrm[k]) / nrm[NIN + k]) : 0;
+    const uint16_t hi = k + 1 < NIN ? f32_to_bf16((f[k + 1] - nrm[k + 1]) / nrm[NIN + k + 1]) : 0;
+    *reinterpret_cast<uint32_t*>(x + k) = (uint32_t)lo | ((uint32_t)hi << 16);
+  }
 }
 
-void launch_mlp_features(long V, const MatParams& P, const Orient<12>* orients, const int32_t* oid,
+template <int NS>
+void launch_mlp_features(long V, const MatParams& P, const Orient<NS>* orients, const int32_t* oid,
                          int step, const double* Fn, const double* Fn1, const double* sn,
                          const double* snp1, long e0, long stride, long nrows, float* X, float* T,
                          hipStream_t s) {
-  k_mlp_features<<<(unsigned)((nrows + 63) / 64), 64, 0, s>>>(V, P, orients, oid, step, Fn, Fn1, sn,
-                                                              snp1, e0, stride, nrows, X, T);
+  k_mlp_features<NS><<<(unsigned)((nrows + 63) / 64), 64, 0, s>>>(
+      V, P, orients, oid, step, Fn, Fn1, sn, snp1, e0, stride, nrows, X, T);
 }
-void launch_mlp_denorm(long V, long e0, long nrows, const float* Y, const float* nrm, float* ws,
-                       hipStream_t s) {
-  k_mlp_denorm<<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(V, e0, nrows, Y, nrm, ws);
+template <int NS>
+void launch_mlp_features_norm(long V, const MatParams& P, const Orient<NS>* orients,
+                              const int32_t* oid, int step, const double* Fn, const double* Fn1,
+                              const double* sn, const float* nrm, int ninp, uint16_t* Xv,
+                              hipStream_t s) {
+  k_mlp_features_norm<NS><<<(unsigned)((V + 63) / 64), 64, 0, s>>>(V, P, orients, oid, step, Fn,
+                                                                   Fn1, sn, nrm, ninp, Xv);
 }
+#define CPFFT_INST_FEATURES(NS)                                                                  \
+  template void launch_mlp_features<NS>(long, const MatParams&, const Orient<NS>*, const int32_t*, \
+                                        int, const double*, const double*, const double*,         \
+                                        const double*, long, long, long, float*, float*,          \
+                                        hipStream_t);                                             \
+  template void launch_mlp_features_norm<NS>(long, const MatParams&, const Orient<NS>*,           \
+                                             const int32_t*, int, const double*, const double*,   \
+                                             const double*, const float*, int, uint16_t*,         \
+                                             hipStream_t);
+CPFFT_INST_FEATURES(12)
+CPFFT_INST_FEATURES(24)
+#undef CPFFT_INST_FEATURES
 
 template void launch_drive_eps_sig<12>(long, const MatParams&, const Orient<12>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, double*, const float*,
+                                       double*, double*, double*, double*, const float*, int,
                                        long long*, hipStream_t);
 template void launch_drive_eps_sig<24>(long, const MatParams&, const Orient<24>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, double*, const float*,
+                                       double*, double*, double*, double*, const float*, int,
                                        long long*, hipStream_t);
 template void launch_drive_eps_sig<48>(long, const MatParams&, const Orient<48>*, const int32_t*,
                                        int, int, const double*, const double*, const double*,
-                                       double*, double*, double*, double*, const float*,
+                                       double*, double*, double*, double*, const float*, int,
                                        long long*, hipStream_t);
 
 }  // namespace cpfft

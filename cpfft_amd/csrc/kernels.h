@@ -155,6 +155,7 @@ void launch_copy(long n, const double* x, double* y, hipStream_t s);
 void launch_fill9(long V, const double* v9_host, double* x, hipStream_t s);
 // x[0..n) = v (a store, unlike a scaling by zero, which keeps NaN or Inf)
 void launch_fill(long n, double v, double* x, hipStream_t s);
+void launch_fill_f32(long n, float v, float* x, hipStream_t s);
 // initial material state [W][V] (mm10_init_cc_hist0, mm10_a.f:473-573, with
 // mm10_init_voche, :2042-2055): zero, except Rp = I and tau_tilde = tt0
 void launch_state_init(long V, int W, double tt0, double* state, hipStream_t s);
@@ -172,23 +173,30 @@ void launch_k4_dot9(long V, const double* K4, const double* C, double* partials,
 
 // per-voxel update: stress update (state n+1, P, material tangent into
 // tan6 [36][V]) followed by its pull-back to K4 = dP/dF
-// ws: predicted slip increments [12][V] for the warm start, or nullptr
+// ws: predicted slip increments [NS][V] for the warm start, or nullptr;
+// warm_prev != 0: the state n+1 the call overwrites (this step's previous
+// global iterate) is a warm-start candidate too
 template <int NS>
 void launch_drive_eps_sig(long V, const MatParams& P, const Orient<NS>* orients,
                           const int32_t* orient_id, int step, int iter, const double* Fn,
                           const double* Fn1, const double* sn, double* snp1, double* Pn1,
-                          double* tan6, double* K4, const float* ws, long long* counters,
-                          hipStream_t s);
+                          double* tan6, double* K4, const float* ws, int warm_prev,
+                          long long* counters, hipStream_t s);
 
 // warm-start network inputs of voxels e0 + r*stride (r < nrows) into
-// X[r][42]; with T also their converged slip increments (state n+1)
-void launch_mlp_features(long V, const MatParams& P, const Orient<12>* orients, const int32_t* oid,
+// X[r][18 + 2 NS]; with T also their converged slip increments (state n+1)
+template <int NS>
+void launch_mlp_features(long V, const MatParams& P, const Orient<NS>* orients, const int32_t* oid,
                          int step, const double* Fn, const double* Fn1, const double* sn,
                          const double* snp1, long e0, long stride, long nrows, float* X, float* T,
                          hipStream_t s);
-// ws[k][e0 + r] = Y[r][k] * sd_t[k] + mean_t[k]
-void launch_mlp_denorm(long V, long e0, long nrows, const float* Y, const float* nrm, float* ws,
-                       hipStream_t s);
+// every voxel's inputs normalised with nrm ([mean | sd]) as BF16 rows of
+// ninp columns (zero-padded) into Xv
+template <int NS>
+void launch_mlp_features_norm(long V, const MatParams& P, const Orient<NS>* orients,
+                              const int32_t* oid, int step, const double* Fn, const double* Fn1,
+                              const double* sn, const float* nrm, int ninp, uint16_t* Xv,
+                              hipStream_t s);
 
 int reduce_partials_count(long n);
 

@@ -1,6 +1,7 @@
 // mlp.hip -- the learned warm start of the per-voxel Newton solve: a
-// 42 -> 128 -> 128 -> 12 tanh network trained and evaluated with BF16
-// matrix-core (MFMA) GEMMs and FP32 accumulation.  It has no counterpart in
+// nin -> 128 -> 128 -> nout tanh network (nin = 18 + 2 ns, nout = ns for a
+// crystal of ns slip systems) trained and evaluated with BF16 matrix-core
+// (MFMA) GEMMs and FP32 accumulation.  It has no counterpart in
 // the reference; it only supplies the starting point of mm10_solve
 // (mm10_a.f:2945-3023), whose FP64 Newton iteration and tolerances are
 // unchanged.
@@ -168,31 +169,21 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 
 // one training batch: rows drawn with replacement from the log by a hash of
 // (seed, step, row), normalised; inputs as BF16, targets as FP32
-__global__ void k_gather_batch(int Bn, long count, uint32_t seed, uint32_t step,
-                               const float* __restrict__ logX, const float* __restrict__ logT,
-                               const float* __restrict__ nrm, uint16_t* __restrict__ Xb,
-                               float* __restrict__ Tb) {
+__global__ void k_gather_batch(int Bn, long count, uint32_t seed, uint32_t step, int nin,
+                               int nout, const float* __restrict__ logX,
+                               const float* __restrict__ logT, const float* __restrict__ nrm,
+                               uint16_t* __restrict__ Xb, float* __restrict__ Tb) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= Bn) return;
   const uint32_t h = mix32(seed * 0x9e3779b9u ^ mix32(step * 0x85ebca6bu ^ (uint32_t)r));
   const long src = (long)(h % (uint32_t)count);
   const float* mx = nrm;
-  const float* sx = nrm + kMlpIn;
-  const float* mt = nrm + 2 * kMlpIn;
-  const float* st = nrm + 2 * kMlpIn + kMlpOut;
-  for (int c = 0; c < kMlpIn; ++c)
-    Xb[(long)r * kMlpIn + c] = f2bf((logX[src * kMlpIn + c] - mx[c]) / sx[c]);
-  for (int c = 0; c < kMlpOut; ++c) Tb[(long)r * kMlpOut + c] = (logT[src * kMlpOut + c] - mt[c]) / st[c];
-}
-
-// raw features [rows][42] -> normalised BF16
-__global__ void k_norm_inputs(long rows, const float* __restrict__ X, const float* __restrict__ nrm,
-                              uint16_t* __restrict__ Xb) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < rows * kMlpIn;
-       i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % kMlpIn);
-    Xb[i] = f2bf((X[i] - nrm[c]) / nrm[kMlpIn + c]);
-  }
+  const float* sx = nrm + nin;
+  const float* mt = nrm + 2 * nin;
+  const float* st = nrm + 2 * nin + nout;
+  for (int c = 0; c < nin; ++c)
+    Xb[(long)r * nin + c] = f2bf((logX[src * nin + c] - mx[c]) / sx[c]);
+  for (int c = 0; c < nout; ++c) Tb[(long)r * nout + c] = (logT[src * nout + c] - mt[c]) / st[c];
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights
@@ -220,68 +211,213 @@ void mlp_to_bf16(long n, const float* x, uint16_t* y, hipStream_t s) {
 }
 
 void mlp_weights_to_bf16(const MlpBuffers& b, hipStream_t s) {
-  k_to_bf16<<<grid1(kMlpParams), 256, 0, s>>>(kMlpParams, b.params, b.pbf);
+  k_to_bf16<<<grid1(b.d.nparam), 256, 0, s>>>(b.d.nparam, b.params, b.pbf);
 }
 
 void mlp_forward(const MlpBuffers& b, int rows, const uint16_t* Xb, float* Y, hipStream_t s) {
   const float* P = b.params;
   const uint16_t* Pb = b.pbf;
+  const MlpDims& d = b.d;
   // Z1 = X W1^T + b1, H1 = tanh(Z1)
-  gemm<1>(rows, kMlpHid, kMlpIn, Xb, kMlpIn, 1, Pb + kOffW1, 1, kMlpIn, b.H1, kMlpHid, P + kOffB1,
+  gemm<1>(rows, kMlpHid, d.nin, Xb, d.nin, 1, Pb + d.offW1, 1, d.nin, b.H1, kMlpHid, P + d.offB1,
           nullptr, b.H1b, s);
-  gemm<1>(rows, kMlpHid, kMlpHid, b.H1b, kMlpHid, 1, Pb + kOffW2, 1, kMlpHid, b.H2, kMlpHid,
-          P + kOffB2, nullptr, b.H2b, s);
-  gemm<2>(rows, kMlpOut, kMlpHid, b.H2b, kMlpHid, 1, Pb + kOffW3, 1, kMlpHid, Y, kMlpOut,
-          P + kOffB3, nullptr, nullptr, s);
+  gemm<1>(rows, kMlpHid, kMlpHid, b.H1b, kMlpHid, 1, Pb + d.offW2, 1, kMlpHid, b.H2, kMlpHid,
+          P + d.offB2, nullptr, b.H2b, s);
+  gemm<2>(rows, d.nout, kMlpHid, b.H2b, kMlpHid, 1, Pb + d.offW3, 1, kMlpHid, Y, d.nout,
+          P + d.offB3, nullptr, nullptr, s);
 }
 
 void mlp_backward(const MlpBuffers& b, int rows, const uint16_t* Xb, const float* T, float* grads,
                   double* part, hipStream_t s) {
   const uint16_t* Pb = b.pbf;
-  const long ny = (long)rows * kMlpOut;
+  const MlpDims& d = b.d;
+  const long ny = (long)rows * d.nout;
   const unsigned gm = grid1(ny);
   k_mse<<<gm, 256, 0, s>>>(ny, b.Y, T, b.dY, b.dYb, part);
   // output layer
-  gemm<0>(kMlpOut, kMlpHid, rows, b.dYb, 1, kMlpOut, b.H2b, kMlpHid, 1, grads + kOffW3, kMlpHid,
+  gemm<0>(d.nout, kMlpHid, rows, b.dYb, 1, d.nout, b.H2b, kMlpHid, 1, grads + d.offW3, kMlpHid,
           nullptr, nullptr, nullptr, s);
-  k_colsum<<<kMlpOut, 256, 0, s>>>(rows, kMlpOut, b.dY, grads + kOffB3);
+  k_colsum<<<d.nout, 256, 0, s>>>(rows, d.nout, b.dY, grads + d.offB3);
   // dZ2 = (dY W3) (1 - H2^2)
-  gemm<3>(rows, kMlpHid, kMlpOut, b.dYb, kMlpOut, 1, Pb + kOffW3, kMlpHid, 1, b.dZ2, kMlpHid,
+  gemm<3>(rows, kMlpHid, d.nout, b.dYb, d.nout, 1, Pb + d.offW3, kMlpHid, 1, b.dZ2, kMlpHid,
           nullptr, b.H2, b.dZ2b, s);
-  gemm<0>(kMlpHid, kMlpHid, rows, b.dZ2b, 1, kMlpHid, b.H1b, kMlpHid, 1, grads + kOffW2, kMlpHid,
+  gemm<0>(kMlpHid, kMlpHid, rows, b.dZ2b, 1, kMlpHid, b.H1b, kMlpHid, 1, grads + d.offW2, kMlpHid,
           nullptr, nullptr, nullptr, s);
-  k_colsum<<<kMlpHid, 256, 0, s>>>(rows, kMlpHid, b.dZ2, grads + kOffB2);
+  k_colsum<<<kMlpHid, 256, 0, s>>>(rows, kMlpHid, b.dZ2, grads + d.offB2);
   // dZ1 = (dZ2 W2) (1 - H1^2)
-  gemm<3>(rows, kMlpHid, kMlpHid, b.dZ2b, kMlpHid, 1, Pb + kOffW2, kMlpHid, 1, b.dZ1, kMlpHid,
+  gemm<3>(rows, kMlpHid, kMlpHid, b.dZ2b, kMlpHid, 1, Pb + d.offW2, kMlpHid, 1, b.dZ1, kMlpHid,
           nullptr, b.H1, b.dZ1b, s);
-  gemm<0>(kMlpHid, kMlpIn, rows, b.dZ1b, 1, kMlpHid, Xb, kMlpIn, 1, grads + kOffW1, kMlpIn,
+  gemm<0>(kMlpHid, d.nin, rows, b.dZ1b, 1, kMlpHid, Xb, d.nin, 1, grads + d.offW1, d.nin,
           nullptr, nullptr, nullptr, s);
-  k_colsum<<<kMlpHid, 256, 0, s>>>(rows, kMlpHid, b.dZ1, grads + kOffB1);
+  k_colsum<<<kMlpHid, 256, 0, s>>>(rows, kMlpHid, b.dZ1, grads + d.offB1);
 }
 
-int mlp_loss_parts(int rows) { return (int)grid1((long)rows * kMlpOut); }
+int mlp_loss_parts(const MlpBuffers& b, int rows) { return (int)grid1((long)rows * b.d.nout); }
 
 void mlp_adam(const MlpBuffers& b, float lr, long t, hipStream_t s) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
-  k_adam<<<grid1(kMlpParams), 256, 0, s>>>(kMlpParams, b.params, b.grads, b.m, b.v, lr, b1, b2, eps,
-                                           c1, c2);
+  k_adam<<<grid1(b.d.nparam), 256, 0, s>>>(b.d.nparam, b.params, b.grads, b.m, b.v, lr, b1, b2,
+                                           eps, c1, c2);
 }
 
 void mlp_gather_batch(const MlpBuffers& b, int rows, long count, uint32_t seed, uint32_t step,
                       hipStream_t s) {
-  k_gather_batch<<<(rows + 255) / 256, 256, 0, s>>>(rows, count, seed, step, b.logX, b.logT, b.norm,
-                                                    b.Xb, b.Tb);
-}
-
-void mlp_norm_inputs(const MlpBuffers& b, long rows, const float* X, uint16_t* Xb, hipStream_t s) {
-  k_norm_inputs<<<grid1(rows * kMlpIn), 256, 0, s>>>(rows, X, b.norm, Xb);
+  k_gather_batch<<<(rows + 255) / 256, 256, 0, s>>>(rows, count, seed, step, b.d.nin, b.d.nout,
+                                                    b.logX, b.logT, b.norm, b.Xb, b.Tb);
 }
 
 void mlp_stats(const MlpBuffers& b, long count, hipStream_t s) {
-  k_colstats<<<kMlpIn, 256, 0, s>>>(count, kMlpIn, b.logX, b.norm, b.norm + kMlpIn);
-  k_colstats<<<kMlpOut, 256, 0, s>>>(count, kMlpOut, b.logT, b.norm + 2 * kMlpIn,
-                                     b.norm + 2 * kMlpIn + kMlpOut);
+  const MlpDims& d = b.d;
+  k_colstats<<<d.nin, 256, 0, s>>>(count, d.nin, b.logX, b.norm, b.norm + d.nin);
+  k_colstats<<<d.nout, 256, 0, s>>>(count, d.nout, b.logT, b.norm + 2 * d.nin,
+                                    b.norm + 2 * d.nin + d.nout);
+}
+
+// ------------------------------------------------------------ inference
+// One launch over all voxels.  The block stages the BF16 weights (rows
+// padded by eight elements against LDS bank conflicts) and the FP32 biases
+// and output normalisation in LDS; each wavefront then takes 16 voxels at a
+// time through the three layers with v_mfma_f32_16x16x32_bf16 (lane l
+// supplies A row l%16 and B column l%16 at k = 8*(l/16)..+7 of each K step
+// of 32, and holds C rows 4*(l/16)..+3 of column l%16).  Layer outputs go
+// through a per-wavefront LDS tile (16 x 128 BF16) that the next layer reads
+// as its A operand.  The rounding points are those of mlp_forward: FP32
+// bias and tanh, BF16 hidden activations, FP32 output.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+static constexpr int kIW = 8;               // wavefronts per inference block
+static constexpr int kHS = kMlpHid + 8;     // padded row of a hidden tile / W2, W3 row
+
+template <int NINP, int NOUTP>
+__global__ __launch_bounds__(64 * kIW) void k_mlp_infer(long V, int nin, int nout,
+                                                        const uint16_t* __restrict__ Xv,
+                                                        const uint16_t* __restrict__ pbf,
+                                                        const float* __restrict__ params,
+                                                        MlpDims d, const float* __restrict__ nrm,
+                                                        float* __restrict__ ws) {
+  constexpr int S1 = NINP + 8;
+  __shared__ __align__(16) uint16_t W1[kMlpHid * S1];
+  __shared__ __align__(16) uint16_t W2[kMlpHid * kHS];
+  __shared__ __align__(16) uint16_t W3[NOUTP * kHS];
+  __shared__ __align__(16) uint16_t H[kIW][2][16 * kHS];
+  __shared__ float b1[kMlpHid], b2[kMlpHid], b3[NOUTP], mt[NOUTP], st[NOUTP];
+  const int T = 64 * kIW;
+  for (int i = threadIdx.x; i < kMlpHid * S1; i += T) {
+    const int n = i / S1, k = i - n * S1;
+    W1[i] = k < nin ? pbf[d.offW1 + (long)n * nin + k] : 0;
+  }
+  for (int i = threadIdx.x; i < kMlpHid * kHS; i += T) {
+    const int n = i / kHS, k = i - n * kHS;
+    W2[i] = k < kMlpHid ? pbf[d.offW2 + (long)n * kMlpHid + k] : 0;
+  }
+  for (int i = threadIdx.x; i < NOUTP * kHS; i += T) {
+    const int n = i / kHS, k = i - n * kHS;
+    W3[i] = (n < nout && k < kMlpHid) ? pbf[d.offW3 + (long)n * kMlpHid + k] : 0;
+  }
+  for (int i = threadIdx.x; i < kMlpHid; i += T) {
+    b1[i] = params[d.offB1 + i];
+    b2[i] = params[d.offB2 + i];
+  }
+  for (int i = threadIdx.x; i < NOUTP; i += T) {
+    b3[i] = i < nout ? params[d.offB3 + i] : 0.f;
+    mt[i] = i < nout ? nrm[2 * nin + i] : 0.f;
+    st[i] = i < nout ? nrm[2 * nin + nout + i] : 0.f;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  uint16_t* H1 = H[w][0];
+  uint16_t* H2 = H[w][1];
+  // per-wavefront exchange through LDS: every lane's writes complete before
+  // any lane reads
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  const long nrb = (V + 15) / 16;
+  for (long rb = (long)blockIdx.x * kIW + w; rb < nrb; rb += (long)gridDim.x * kIW) {
+    const long r0 = rb * 16;
+    // layer 1: A = the 16 voxels' inputs straight from HBM
+    bf16x8 a1[NINP / 32];
+    {
+      const long r = r0 + li;
+#pragma unroll
+      for (int ks = 0; ks < NINP / 32; ++ks) {
+        if (r < V) {
+          a1[ks] = *reinterpret_cast<const bf16x8*>(Xv + r * NINP + ks * 32 + kg * 8);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) a1[ks][j] = (__bf16)0.0f;
+        }
+      }
+    }
+#pragma unroll
+    for (int jt = 0; jt < kMlpHid / 16; ++jt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < NINP / 32; ++ks) {
+        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(W1 + (16 * jt + li) * S1 + ks * 32 + kg * 8);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[ks], bw, acc, 0, 0, 0);
+      }
+      const int n = 16 * jt + li;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) H1[(4 * kg + j) * kHS + n] = f2bf(tanhf(acc[j] + b1[n]));
+    }
+    wave_sync();
+    // layer 2
+    bf16x8 a2[kMlpHid / 32];
+#pragma unroll
+    for (int ks = 0; ks < kMlpHid / 32; ++ks)
+      a2[ks] = *reinterpret_cast<const bf16x8*>(H1 + li * kHS + ks * 32 + kg * 8);
+#pragma unroll
+    for (int jt = 0; jt < kMlpHid / 16; ++jt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kMlpHid / 32; ++ks) {
+        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(W2 + (16 * jt + li) * kHS + ks * 32 + kg * 8);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw, acc, 0, 0, 0);
+      }
+      const int n = 16 * jt + li;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) H2[(4 * kg + j) * kHS + n] = f2bf(tanhf(acc[j] + b2[n]));
+    }
+    wave_sync();
+    // layer 3 and the output normalisation
+#pragma unroll
+    for (int ks = 0; ks < kMlpHid / 32; ++ks)
+      a2[ks] = *reinterpret_cast<const bf16x8*>(H2 + li * kHS + ks * 32 + kg * 8);
+#pragma unroll
+    for (int jt = 0; jt < NOUTP / 16; ++jt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kMlpHid / 32; ++ks) {
+        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(W3 + (16 * jt + li) * kHS + ks * 32 + kg * 8);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw, acc, 0, 0, 0);
+      }
+      const int n = 16 * jt + li;
+      if (n < nout) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const long r = r0 + 4 * kg + j;
+          if (r < V) ws[n * V + r] = (acc[j] + b3[n]) * st[n] + mt[n];
+        }
+      }
+    }
+    // the next row block rewrites H1, H2: all lanes are done reading them
+    wave_sync();
+  }
+}
+
+void mlp_infer(const MlpBuffers& b, long V, hipStream_t s) {
+  const unsigned g = 256;  // one block per CU (the LDS holds one); each loops over row blocks
+  const MlpDims& d = b.d;
+  if (mlp_nin_pad(d.nin) == 64 && d.nout <= 16)
+    k_mlp_infer<64, 16><<<g, 64 * kIW, 0, s>>>(V, d.nin, d.nout, b.Xv, b.pbf, b.params, d, b.norm,
+                                                b.ws);
+  else
+    k_mlp_infer<96, 32><<<g, 64 * kIW, 0, s>>>(V, d.nin, d.nout, b.Xv, b.pbf, b.params, d, b.norm,
+                                                b.ws);
 }
 
 }  // namespace cpfft

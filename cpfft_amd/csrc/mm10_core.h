@@ -570,10 +570,12 @@ CPFFT_HD inline void setup_point(const MatParams& P, const double R[3][3], const
 template <int NS>
 CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const Point& pt,
                                 const StateN& n, double dtinc, double x[7], double xprev[7],
-                                double* tt_rate, Counters& cnt, const double* x0w = nullptr) {
+                                double* tt_rate, Counters& cnt, const double* x0w = nullptr,
+                                double x0w_res = 0.0) {
   const double cls = 1.0e-4, red = 0.5;
   const int mls = 10;
   int fail = 0;
+  bool warm = false;
   double inR1;
   // ---- predictor: 6x6 Newton on R1 at extrapolated tt (mm10_a.f:2975-3156)
   {
@@ -608,28 +610,16 @@ CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const P
     for (int k = 0; k < 6; ++k) dot += R1[k] * R1[k];
     double nR1 = sqrt(dot);
     inR1 = nR1;
-    if (x0w) {
-      // warm start: iterate from x0w when its residual is below the plain
-      // start's; the plain start's residual stays the reference of the
-      // relative stop test, so the solve stops at the accuracy the plain
-      // method asks for
-      double R1w[6], sabsw, dotw = 0.0;
-      form_R1<NS>(P, o, pt, n, x0w, x0w[6], R1w, &sabsw);
+    if (x0w && x0w_res < nR1) {
+      // warm start: the full state x0w (stress and slip resistance) has a
+      // full residual below the plain start's predictor residual, so the
+      // predictor is skipped and the full Newton starts there
 #pragma unroll
-      for (int k = 0; k < 6; ++k) dotw += R1w[k] * R1w[k];
-      if (dotw < dot) {
-        x2 = x0w[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          x1[k] = x0w[k];
-          R1[k] = R1w[k];
-        }
-        dot = dotw;
-        nR1 = sqrt(dot);
-      }
+      for (int k = 0; k < 7; ++k) x[k] = x0w[k];
+      warm = true;
     }
     int it = 0, pfail = 0;
-    while (nR1 > P.atol1 && nR1 / inR1 > P.rtol1) {
+    while (!warm && nR1 > P.atol1 && nR1 / inR1 > P.rtol1) {
       double A[6][6], dx[6][1], gJ[6];
       form_J<NS, 6>(P, o, pt, x1, x2, A);
       // line-search slope J^T R before A is factorised
@@ -678,7 +668,7 @@ CPFFT_HD inline int local_solve(const MatParams& P, const Orient<NS>& o, const P
     cnt.pred_it += it;
     if (pfail) {
       fail = 1;
-    } else {
+    } else if (!warm) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) x[k] = x1[k];
       x[6] = x2;
@@ -1034,14 +1024,14 @@ CPFFT_HD inline void warm_x0(const MatParams& P, const Orient<NS>& o, const Poin
   x0[6] = tt0;
 }
 
-// Inputs of the warm-start network for one voxel (42 values, FCC): the
-// trial stress s_n + C:D rotated to the lattice frame (Rp_n^T S Rp_n as a
-// full 3x3, row-major), the slip resistance tt_n once per slip system (the
-// Voce law keeps a single one), the strain increment D in the lattice frame
-// (3x3 tensor, row-major), and the slip increments of the previous step
-// (as u_s, see slip_to_u).
-template <class LoadF, class LoadS>
-CPFFT_HD inline void voxel_features(const MatParams& P, const Orient<12>& o, int step, LoadF&& ldF,
+// Inputs of the warm-start network for one voxel (18 + 2 NS values: 42 for
+// fcc, 66 for bcc24): the trial stress s_n + C:D rotated to the lattice
+// frame (Rp_n^T S Rp_n as a full 3x3, row-major), the slip resistance tt_n
+// once per slip system (the Voce law keeps a single one), the strain
+// increment D in the lattice frame (3x3 tensor, row-major), and the slip
+// increments of the previous step (as u_s, see slip_to_u).
+template <int NS, class LoadF, class LoadS>
+CPFFT_HD inline void voxel_features(const MatParams& P, const Orient<NS>& o, int step, LoadF&& ldF,
                                     LoadS&& ldn, float* feat) {
   double uddt[6], rn1[3][3];
   {
@@ -1095,26 +1085,31 @@ CPFFT_HD inline void voxel_features(const MatParams& P, const Orient<12>& o, int
           b += Rp[k][i] * D[k][l] * Rp[l][j];
         }
       feat[3 * i + j] = (float)a;
-      feat[21 + 3 * i + j] = (float)b;
+      feat[9 + NS + 3 * i + j] = (float)b;
     }
 #pragma unroll
-  for (int s = 0; s < 12; ++s) feat[9 + s] = (float)tt;
+  for (int s = 0; s < NS; ++s) feat[9 + s] = (float)tt;
   const double dg = P.alter_mode ? P.eps_dot_0_y * P.dt : 1.0;
-  for (int s = 0; s < 12; ++s) feat[30 + s] = step == 1 ? 0.0f : (float)slip_to_u(P, dg, ldn(23 + s));
+  for (int s = 0; s < NS; ++s)
+    feat[18 + NS + s] = step == 1 ? 0.0f : (float)slip_to_u(P, dg, ldn(23 + s));
 }
 
 // Per-voxel stress update.  ldF(0|1, 3i+j) loads F_n | F_{n+1}; ldn(w)
 // loads state-n word w; stn1(w, v) stores state n+1; stP(3i+j, v) stores
 // P; stT(6a+b, v) stores the symmetrised material tangent (row-major 6x6)
 // that voxel_dPdF pulls back to dP/dF.
-// ws, if not null, holds predicted slip increments ws[s * wstride] of this
-// voxel: the plastic solve then starts from warm_x0 and, if that solve
-// fails, is redone from the plain start.
+// Warm start (off unless ws or xprev_it is given): ws holds predicted slip
+// increments ws[s * wstride] of this voxel (warm_x0 turns them into a full
+// state), xprev_it the stress and slip resistance (7 values) of this step's
+// previous global iteration.  The candidate of smaller full residual is
+// handed to local_solve, which starts its full Newton there when that
+// residual is below the plain start's predictor residual; a warm solve that
+// fails is redone from the plain start.
 template <int NS, class LoadF, class LoadS, class StoreS, class StoreP, class StoreT>
 CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int step, int iter,
                                   LoadF&& ldF, LoadS&& ldn, StoreS&& stn1, StoreP&& stP,
                                   StoreT&& stT, Counters& cnt, const float* ws = nullptr,
-                                  long wstride = 0) {
+                                  long wstride = 0, const double* xprev_it = nullptr) {
   double uddt[6], rn1[3][3];
   {
     double Fn[3][3], Fn1[3][3], rnh[3][3], fnhinv[3][3];
@@ -1195,11 +1190,24 @@ CPFFT_HD inline void voxel_update(const MatParams& P, const Orient<NS>& o, int s
     for (int k = 0; k < 6; ++k) { x[k] = n.stress[k]; ost[k] = x[k]; }
     double frac = 0.0, stp = 1.0;
     int cuts = 0, fail = 0;
-    if (ws) {
-      double x0[7];
-      warm_x0<NS>(P, o, pt, n, ws, wstride, x0);
+    if (ws || xprev_it) {
+      double x0[7], xc[7], R[7], tr, best = 1.0e300;
+      auto consider = [&](const double* c) {
+        const double d = sqrt(form_R<NS>(P, o, pt, n, c, R, &tr));
+        if (d < best) {  // false for NaN
+          best = d;
+#pragma unroll
+          for (int k = 0; k < 7; ++k) x0[k] = c[k];
+        }
+      };
+      if (ws) {
+        warm_x0<NS>(P, o, pt, n, ws, wstride, xc);
+        consider(xc);
+      }
+      if (xprev_it) consider(xprev_it);
       x[6] = n.tt;
-      fail = local_solve<NS>(P, o, pt, n, P.dt, x, xprev, &ttrate1, cnt, x0);
+      fail = local_solve<NS>(P, o, pt, n, P.dt, x, xprev, &ttrate1, cnt,
+                             best < 1.0e300 ? x0 : nullptr, best);
       bool ok = !fail && (x[6] == x[6]);
 #pragma unroll
       for (int k = 0; k < 6; ++k) ok &= (x[k] == x[k]);

@@ -167,6 +167,7 @@ contains
     do istep = 1, nstep
       step = max(first_step, 1) + istep - 1
       if (verbose /= 0) write(*, 1000) step
+      if (verbose /= 0) flush(6)
       rc = cpfft_counters(ctx, cnt0)
       nnewton = 0
       ncalls = 0
@@ -210,6 +211,7 @@ contains
         rc = cpfft_dnrm2(ctx, dFm, dn)
         resfft = dn / Fnorm
         if (verbose /= 0) write(*, 1003) resfft
+        if (verbose /= 0) flush(6)
         iiter_EBC = 0
         resfft = 1.0d0          ! at least one iteration (FFT_nr3.f:103)
 
@@ -235,6 +237,7 @@ contains
           rc = cpfft_dnrm2(ctx, dFm, dn)
           resfft = dn / Fnorm
           if (verbose /= 0) write(*, 1001) iiter_EBC, resfft
+          if (verbose /= 0) flush(6)
           if (iiter_EBC == maxIter) then
             cpfft_compute = -4
             return
@@ -268,6 +271,7 @@ contains
         end if
         nloops = nloops + 1
         if (verbose /= 0) write(*, 1002) iiter_NBC, resPbar(3)
+        if (verbose /= 0) flush(6)
         if (resPbar(3) <= tolNR) exit
         if (iiter_NBC > maxIter) then
           cpfft_compute = -5

@@ -98,6 +98,9 @@ double* cpfft_field(cpfft_ctx* ctx, int field);
 /* host <-> device copies of whole fields */
 int cpfft_upload(cpfft_ctx* ctx, int field, const double* host);
 int cpfft_download(cpfft_ctx* ctx, int field, double* host);
+/* voxels [e0, e0+n) of every component of a field, host[k][i] =
+ * field[k][e0+i] (a slab of a large field without copying all of it)     */
+int cpfft_download_voxels(cpfft_ctx* ctx, int field, long e0, long n, double* host);
 /* reset Fn = Fn1 = I, Pn = Pn1 = 0, state = initial (FFT_init.f:153-160) */
 int cpfft_reset(cpfft_ctx* ctx);
 int cpfft_sync(cpfft_ctx* ctx);
@@ -238,21 +241,31 @@ int cpfft_kernel_times(cpfft_ctx* ctx, double out[8]);
 int cpfft_set_timing(cpfft_ctx* ctx, int on);
 
 /* ---- learned warm start ------------------------------------------------
- * No counterpart in the reference.  A 42 -> 128 -> 128 -> 12 tanh network,
- * trained and evaluated with BF16 matrix-core GEMMs (FP32 accumulation),
- * predicts each voxel's slip increments from its trial state.  The
- * prediction only sets the starting point of the local Newton solve
- * (mm10_a.f:2945-3023): the FP64 iteration and its tolerances are
+ * No counterpart in the reference.  An nin -> 128 -> 128 -> nout tanh
+ * network, trained and evaluated with BF16 matrix-core GEMMs (FP32
+ * accumulation), predicts each voxel's slip increments from its trial
+ * state.  The prediction only sets the starting point of the local Newton
+ * solve (mm10_a.f:2945-3023): the FP64 iteration and its tolerances are
  * unchanged, and a warm-started solve that fails is redone from the plain
- * start.  FCC (12 slip systems) only.
- * Inputs (42): trial stress s_n + C:D in the lattice frame (3x3), the slip
- * resistance once per system (12), the lattice-frame strain increment
- * (3x3), the previous slip increments (12).  Outputs: slip increments.   */
-#define CPFFT_MLP_IN 42
+ * start.  For fcc (ns = 12 slip systems) and bcc24 (ns = 24).
+ * Inputs (nin = 18 + 2 ns: 42 for fcc, 66 for bcc24): trial stress s_n + C:D
+ * in the lattice frame (3x3), the slip resistance once per system (ns), the
+ * lattice-frame strain increment (3x3), the previous slip increments (ns).
+ * Outputs (nout = ns): slip increments.  Sizes as functions of ns:        */
 #define CPFFT_MLP_HID 128
+#define CPFFT_MLP_NIN(ns) (18 + 2 * (ns))
+#define CPFFT_MLP_NOUT(ns) (ns)
+/* W1 (128 x nin) b1 W2 (128 x 128) b2 W3 (nout x 128) b3, rows = outputs  */
+#define CPFFT_MLP_NPARAMS(ns) \
+  (CPFFT_MLP_HID * (CPFFT_MLP_NIN(ns) + 1) + CPFFT_MLP_HID * (CPFFT_MLP_HID + 1) + \
+   (ns) * (CPFFT_MLP_HID + 1))
+/* mean_x[nin] sd_x[nin] mean_t[nout] sd_t[nout]                            */
+#define CPFFT_MLP_NNORMS(ns) (2 * CPFFT_MLP_NIN(ns) + 2 * (ns))
+/* the fcc sizes                                                            */
+#define CPFFT_MLP_IN 42
 #define CPFFT_MLP_OUT 12
-#define CPFFT_MLP_NPARAM 23564 /* W1 (128x42) b1 W2 (128x128) b2 W3 (12x128) b3, rows = outputs */
-#define CPFFT_MLP_NNORM 108    /* mean_x[42] sd_x[42] mean_t[12] sd_t[12]                    */
+#define CPFFT_MLP_NPARAM 23564
+#define CPFFT_MLP_NNORM 108
 /* allocate the network (Glorot-uniform weights from seed), a log of
  * `capacity` training pairs and work buffers for batches of `batch` rows  */
 int cpfft_mlp_init(cpfft_ctx* ctx, uint64_t seed, long capacity, int batch);
@@ -265,17 +278,36 @@ int cpfft_mlp_log(cpfft_ctx* ctx, int step, long stride);
  * drawn with replacement; *loss = mean squared error (normalised units) of
  * the last batch                                                          */
 int cpfft_mlp_train(cpfft_ctx* ctx, int iters, double lr, uint64_t seed, double* loss);
-/* on != 0: drive_eps_sig calls with iiter > 0 warm-start every voxel      */
+/* Warm start of the local solves of drive_eps_sig calls with iiter > 0
+ * (no counterpart in the reference; off by default, when results are those
+ * of the reference's start).  mode is a bit set of the candidate starts:
+ * CPFFT_WARM_NETWORK the network's predicted slip increments (needs
+ * cpfft_mlp_init; fcc and bcc24), CPFFT_WARM_PREVIOUS the step's previous
+ * global iterate held in STATE_NP1.  Per voxel the candidate of smaller
+ * full residual replaces the reference's predictor stage when its residual
+ * is below the predictor's starting residual; the Newton tolerances are
+ * unchanged and a failed warm solve is redone from the reference's start.
+ * Returns -1 on unknown bits or a network start without a network.       */
+#define CPFFT_WARM_NETWORK 1
+#define CPFFT_WARM_PREVIOUS 2
+int cpfft_set_warm_start(cpfft_ctx* ctx, int mode);
+/* on != 0 adds CPFFT_WARM_NETWORK to the warm-start mode, on == 0 removes it */
 int cpfft_mlp_warm_start(cpfft_ctx* ctx, int on);
-/* parameters then normalisation (CPFFT_MLP_NPARAM + CPFFT_MLP_NNORM
+/* parameters then normalisation (CPFFT_MLP_NPARAMS(ns) + CPFFT_MLP_NNORMS(ns)
  * floats): get != 0 copies them out, get == 0 sets them                   */
 int cpfft_mlp_params(cpfft_ctx* ctx, float* buf, int get);
-/* network output Y[n][12] for normalised inputs Xn[n][42] (n <= batch)    */
+/* network output Y[n][nout] for normalised inputs Xn[n][nin] (n <= batch) */
 int cpfft_mlp_forward(cpfft_ctx* ctx, const float* Xn, int n, float* Y);
-/* gradient (CPFFT_MLP_NPARAM floats) and value of mean((Y - Tn)^2) over a
- * batch of normalised inputs and targets (n <= batch)                     */
+/* gradient (CPFFT_MLP_NPARAMS(ns) floats) and value of mean((Y - Tn)^2)
+ * over a batch of normalised inputs and targets (n <= batch)              */
 int cpfft_mlp_grad(cpfft_ctx* ctx, const float* Xn, const float* Tn, int n, float* grad,
                    double* loss);
+/* the warm start of every voxel for the current state (FN, FN1, STATE_N)
+ * of load step `step`: X (if not NULL) receives the raw network inputs
+ * [V][nin], ws (if not NULL) the predicted slip increments, as the
+ * normalised resolved shears the local solve starts from, [nout][V] --
+ * what cpfft_drive_eps_sig uses when the warm start is on                 */
+int cpfft_mlp_predict(cpfft_ctx* ctx, int step, float* X, float* ws);
 /* [0] logged pairs, [1] training steps, [2] warm-started voxel updates,
  * [3] warm-started solves redone from the plain start                     */
 int cpfft_mlp_stats(cpfft_ctx* ctx, long long out[4]);

@@ -102,12 +102,15 @@ void orc_ddot42n(long V, const double* A4, const double* B2, double* C2);
 int  orc_pcg(orc_fft* f, const double* K4, const double* b, double* x, double tol,
              int* iters, double* relres);
 int  orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]);
+int  orc_tangent_homo_it(orc_fft* f, const double* K4, double tol, double C_homo[81],
+                         int* iters);
 int  orc_nbc_update(const double C_homo[81], double DbarF[9], const double P_bar[9],
                     const double PBC[9], const int isNBC[9]);
 
 /* full FFT_nr3 loop (FFT_nr3.f:14-200) over nstep load steps.
  * BC_all: [nstep][9]; Pbar_out: [nstep][9]; counters_out: [nstep][4] =
- * (Newton iterations, CG iterations, mixed-BC loops, drive_eps_sig calls). */
+ * (Newton iterations, CG iterations of every solve in the step, tangent_homo's
+ * included, mixed-BC loops, drive_eps_sig calls). */
 typedef struct {
   int N; long V;
   const orc_crystal* cp;

@@ -273,9 +273,12 @@ int orc_pcg(orc_fft* f, const double* K4, const double* b, double* x, double tol
 
 /* tangent_homo (tangent_homo.f:11-73).  Returns 0, or the first negative
  * status of its nine fftPcg solves (fftPcg aborts there, FFT_nr3.f:248-251,
- * 335-339); C_homo is filled either way. */
-int orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]) {
+ * 335-339); C_homo is filled either way.  *iters (if not NULL) receives the
+ * CG iterations of the nine solves. */
+int orc_tangent_homo_it(orc_fft* f, const double* K4, double tol, double C_homo[81],
+                        int* iters) {
   int status = 0;
+  if (iters) *iters = 0;
   long V = f->V, n = 9 * V;
   double* C = (double*)malloc(n * sizeof(double));
   double* b = (double*)malloc(n * sizeof(double));
@@ -286,6 +289,7 @@ int orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]
     int it; double rel;
     int st = orc_pcg(f, K4, b, C, tol, &it, &rel);
     if (st < 0 && status == 0) status = st;
+    if (iters) *iters += it;
     for (long e = 0; e < V; ++e) C[i * V + e] += 1.0;
     for (int j = 0; j < 9; ++j) {
       double s = dot(n, K4 + (long)9 * j * V, C);
@@ -294,6 +298,10 @@ int orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]
   }
   free(C); free(b);
   return status;
+}
+
+int orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]) {
+  return orc_tangent_homo_it(f, K4, tol, C_homo, NULL);
 }
 
 /* NBC_update (FFT_nr3.f:375-433) */
@@ -419,7 +427,11 @@ int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNB
       nbc_loops++;
       if (res3 <= pb->tolNR) break;
       if (iiter_NBC > pb->maxIter) { rc = -5; goto done; }
-      if ((rc = orc_tangent_homo(f, pb->K4, pb->tolPCG, C_homo)) < 0) goto done;
+      /* the step's CG iterations include tangent_homo's (as the library's
+       * iteration counter does) */
+      int hit = 0;
+      if ((rc = orc_tangent_homo_it(f, pb->K4, pb->tolPCG, C_homo, &hit)) < 0) goto done;
+      cgits += hit;
       for (int i = 0; i < 9; ++i) DbarF[i] = 0.0;
       if (orc_nbc_update(C_homo, DbarF, P_bar, PBC, isNBC)) { rc = -3; goto done; }
       for (int i = 0; i < 9; ++i) barF[i] = barF[i] + DbarF[i];

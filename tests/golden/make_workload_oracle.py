@@ -1,0 +1,69 @@
+"""Writes oracle fixtures for the GPU tests of the larger workloads, so the
+tests check full-size runs without minutes of CPU work at test time:
+
+  c2_64_oracle.npz   the first two load steps of the 64^3, 64-grain FCC
+                     workload (configs c2_64: uniaxial tension under mixed
+                     boundary conditions): the inputs, the volume-averaged P
+                     and the Newton / CG / mixed-BC / update counts per
+                     step, and the final P and material state of every
+                     256th voxel
+  c4_n32_oracle.npz  the first two load steps of every 32^3 member of the
+                     batched workload (configs c4_batch: its own grain
+                     count, loading direction and strain rate, every
+                     component of F prescribed): per member its index in
+                     the batch, P per step and the counts per step
+
+The oracle is oracle/ (the CPU restatement of FFT_nr3 and the mm10
+update).  Run from the repository root after building the oracle:
+    python tests/golden/make_workload_oracle.py [c2|c4]
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle_lib as O  # noqa: E402
+from cpfft_amd.configs import BATCH_WORKLOADS, WORKLOADS  # noqa: E402
+
+NSTEP = 2
+
+
+def run(wl, nstep=NSTEP):
+    BC, isNBC = wl.bc()
+    oc = O.crystal(slip_set=wl.material.slip_set, jac_exact=wl.material.jac_exact)
+    ang = wl.angles()
+    P, cnt, rc, st = O.fft_nr3(oc, wl.N, ang, wl.dt, BC[:nstep], isNBC, wl.tolPCG, wl.tolNR,
+                               wl.maxIter, gamma_mode=wl.gamma_mode)
+    assert rc == 0, (wl.name, rc)
+    return BC[:nstep], isNBC, ang, P, cnt, st
+
+
+def c2():
+    wl = WORKLOADS["c2_64"]
+    BC, isNBC, ang, P, cnt, st = run(wl)
+    idx = np.arange(0, wl.N ** 3, 256)
+    np.savez_compressed(os.path.join(HERE, "c2_64_oracle.npz"), BC=BC, isNBC=isNBC, Pbar=P,
+                        counters=cnt, idx=idx, P=st["Pn"][:, idx], state=st["sn"][:, idx])
+    print("c2_64 P_bar_xx:", P[:, 0].tolist(), "counters:", cnt.tolist())
+
+
+def c4():
+    members = BATCH_WORKLOADS["c4_batch"].members()
+    keep = [k for k, m in enumerate(members) if m.N == 32]
+    Ps, cs = [], []
+    for k in keep:
+        _, _, _, P, cnt, _ = run(members[k])
+        Ps.append(P)
+        cs.append(cnt)
+        print(f"c4 member {k}: P_bar_xx {P[:, 0].tolist()} counters {cnt.tolist()}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "c4_n32_oracle.npz"), members=np.array(keep),
+                        Pbar=np.array(Ps), counters=np.array(cs))
+
+
+if __name__ == "__main__":
+    for w in sys.argv[1:] or ["c2", "c4"]:
+        {"c2": c2, "c4": c4}[w]()

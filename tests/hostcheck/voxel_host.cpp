@@ -11,7 +11,8 @@ using namespace cpfft;
 template <int NS>
 static void run(const cpfft_material& m, double dt, long V, const double* ang, int step,
                 int iter, const double* Fn, const double* Fn1, const double* sn, double* snp1,
-                double* Pn1, double* K4, long long* cnt, const float* ws = nullptr) {
+                double* Pn1, double* K4, long long* cnt, const float* ws = nullptr,
+                int warm_prev = 0) {
   double b[kMaxSlip][3], nn[kMaxSlip][3], st[6][6];
   int ns = 0;
   slip_systems(m.slip_set, ns, b, nn);
@@ -24,13 +25,17 @@ static void run(const cpfft_material& m, double dt, long V, const double* ang, i
     const double a[3] = {ang[e], ang[V + e], ang[2 * V + e]};
     build_orient<NS>(st, b, nn, a, o);
     Counters c{0, 0, 0, 0, 0, 0};
-    double tan6[36];
+    double tan6[36], xp[7];
+    if (warm_prev) {  // snp1 holds the previous global iterate on entry
+      for (int k = 0; k < 6; ++k) xp[k] = snp1[k * V + e];
+      xp[6] = snp1[15 * V + e];
+    }
     auto ldF = [&](int which, int k) { return which ? Fn1[k * V + e] : Fn[k * V + e]; };
     voxel_update<NS>(
         P, o, step, iter, ldF, [&](int w) { return sn[w * V + e]; },
         [&](int w, double v) { snp1[w * V + e] = v; },
         [&](int k, double v) { Pn1[k * V + e] = v; }, [&](int k, double v) { tan6[k] = v; }, c,
-        ws ? ws + e : nullptr, V);
+        ws ? ws + e : nullptr, V, warm_prev ? xp : nullptr);
     double t6[6];
     for (int k = 0; k < 6; ++k) t6[k] = snp1[k * V + e];
     voxel_dPdF(ldF, t6, [&](int k) { return tan6[k]; },
@@ -66,6 +71,22 @@ extern "C" int cpfft_host_drive_eps_sig_ws(const cpfft_material* m, double dt, l
                                            const float* ws) {
   if (m->slip_set != SLIP_FCC) return -1;
   run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt, ws);
+  return 0;
+}
+
+// the same update warm-started from the previous global iterate that snp1
+// holds on entry (any slip set)
+extern "C" int cpfft_host_drive_eps_sig_prev(const cpfft_material* m, double dt, long V,
+                                             const double* angles, int step, int iter,
+                                             const double* Fn, const double* Fn1,
+                                             const double* sn, double* snp1, double* Pn1,
+                                             double* K4, long long* cnt) {
+  switch (slip_count(m->slip_set)) {
+    case 12: run<12>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt, nullptr, 1); break;
+    case 24: run<24>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt, nullptr, 1); break;
+    case 48: run<48>(*m, dt, V, angles, step, iter, Fn, Fn1, sn, snp1, Pn1, K4, cnt, nullptr, 1); break;
+    default: return -1;
+  }
   return 0;
 }
 

@@ -33,7 +33,7 @@ def solver(N, mode, ngr=4, seed=7, mat=None, dt=0.5):
     return C.Solver(N, ang, mat or C.Material(), dt=dt, gamma_mode=mode), ang
 
 
-@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32, 48, 64])
+@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32, 48, 64, 96])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_gkdf(N, mode):
     """Generic runtime-plan path (N = 4..12) and compile-time-planned path

@@ -5,6 +5,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
 import mlp_oracle as M  # noqa: E402
@@ -25,18 +26,21 @@ def test_bf16_round_to_nearest_even():
     assert np.array_equal(M.to_bf16(x), want)
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("nin,nout", [(M.IN, M.OUT), (66, 24)])
+def test_gradient_matches_finite_differences(nin, nout):
     rng = np.random.default_rng(5)
-    p = rng.standard_normal(M.NPARAM) * 0.1
-    X = rng.standard_normal((6, M.IN))
-    T = rng.standard_normal((6, M.OUT))
+    w1, b1, w2, b2, w3, b3, end = M.offsets(nin, nout)
+    p = rng.standard_normal(end) * 0.1
+    X = rng.standard_normal((6, nin))
+    T = rng.standard_normal((6, nout))
     loss, g = M.loss_and_grad(p, X, T, bf16=False)
-    idx = np.concatenate([rng.choice(np.arange(M.O_W1, M.O_B1), 6, replace=False),
-                          rng.choice(np.arange(M.O_B1, M.O_W2), 3, replace=False),
-                          rng.choice(np.arange(M.O_W2, M.O_B2), 6, replace=False),
-                          rng.choice(np.arange(M.O_B2, M.O_W3), 3, replace=False),
-                          rng.choice(np.arange(M.O_W3, M.O_B3), 6, replace=False),
-                          np.arange(M.O_B3, M.NPARAM)])
+    assert g.size == end
+    idx = np.concatenate([rng.choice(np.arange(w1, b1), 6, replace=False),
+                          rng.choice(np.arange(b1, w2), 3, replace=False),
+                          rng.choice(np.arange(w2, b2), 6, replace=False),
+                          rng.choice(np.arange(b2, w3), 3, replace=False),
+                          rng.choice(np.arange(w3, b3), 6, replace=False),
+                          np.arange(b3, end)])
     h = 1e-6
     for i in idx:
         pp, pm = p.copy(), p.copy()
