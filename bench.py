@@ -5,11 +5,12 @@ A step is one converged load step of the FFT_nr3 loop (mixed-BC uniaxial
 tension of the 128^3, 256-grain FCC polycrystal), driven by the Fortran
 driver over the HIP operators.  W warm-up steps run first (they include the
 initial elastic call and homogenised tangent), then K steps are timed.
-The headline value is the plain method.  A second solver repeats the run
-with the learned warm start: it logs training pairs during the W warm-up
-steps, trains the network on the GPU, and times the same K steps
-warm-started; its rate and its averaged-stress difference from the plain run
-are reported beside the headline.
+The headline value is the plain method (the reference's local start).
+Further solvers repeat the run with the warm start of the local solves on
+for the K timed steps: from the step's previous global iterate, and (fcc and
+bcc24) from that and the network, trained on the GPU on pairs logged during
+the W warm-up steps; their rates, local iteration counts and averaged-stress
+differences from the plain run are reported beside the headline.
 
 With --workload c4_batch the run is the batched uneven-RVE workload
 instead (24 RVEs of edge 32-96 solved at once; see batch_bench).
@@ -81,14 +82,18 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     V = wl.N ** 3
-    # operator sample: one G_K_dF with K4 at full size on the current state
+    # operator sample: the oracle's fftPcg loop at full size on the current
+    # state, run for a fixed number of iterations (its tolerance is never
+    # met), so the per-iteration time includes the vector updates as in a
+    # whole run; one iteration is one G_K_dF with K4
     K4 = solver.download(C.K4)
-    F = solver.download(C.DFM)
     fo = O.FFT(wl.N, wl.gamma_mode)
+    b = -fo.gkdf(solver.download(C.DFM), K4)
+    nit = max(3, min(40, int(10.0 / (1.6e-7 * V))))  # ~10 s at 16 host threads
     t = time.perf_counter()
-    fo.gkdf(F, K4)
-    t_gkdf = time.perf_counter() - t
-    del K4
+    _, _, it, _ = fo.pcg(K4, b, 1e-15, maxit=nit)
+    t_gkdf = (time.perf_counter() - t) / max(1, it)
+    del K4, b
     # material sample: a contiguous slab of voxels at the current state,
     # plastic update (Newton iteration 1)
     nv = min(V, 16384)
@@ -97,7 +102,8 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
     Fn1 = np.ascontiguousarray(solver.download(C.FN1)[:, sl])
     sn = np.ascontiguousarray(solver.download(C.STATE_N)[:, sl])
     ang = np.ascontiguousarray(wl.angles()[:, sl])
-    oc = O.crystal(jac_exact=wl.material.jac_exact)
+    oc = O.crystal(slip_set=wl.material.slip_set, jac_exact=wl.material.jac_exact)
+    O.drive_eps_sig(oc, ang[:, :256], 3, 1, wl.dt, Fn[:, :256], Fn1[:, :256], sn[:, :256])
     t = time.perf_counter()
     O.drive_eps_sig(oc, ang, 3, 1, wl.dt, Fn, Fn1, sn)
     t_des = (time.perf_counter() - t) * (V / nv)
@@ -111,7 +117,8 @@ def cpu_baseline(wl, solver, per_step, budget_s=20.0):
         "implementation": ("oracle/ C restatement with OpenMP: the reference's complex-to-complex "
                            "transform with phase factors, radix-2 for power-of-two N (not MKL), "
                            "the mm10 update voxel by voxel"),
-        "sample": (f"oracle G_K_dF with K4 once at {wl.N}^3 ({t_gkdf:.2f} s) and oracle "
+        "sample": (f"oracle fftPcg at {wl.N}^3 for {it} iterations ({t_gkdf:.3f} s per "
+                   f"iteration) and oracle "
                    f"drive_eps_sig on {nv} voxels of the current {wl.N}^3 state "
                    f"({t_des * nv / V:.2f} s, x{V / nv:.0f} for all voxels); one load step = "
                    f"{n_gkdf:.0f} G_K_dF + {n_des:.1f} drive_eps_sig calls (this run's counts)"),
@@ -140,6 +147,7 @@ def validate_cpu_baseline(name, nstep):
     print(f"[validate] estimate {est['seconds_per_step']:.1f} s for steps 1-{nstep}; "
           "running the oracle", flush=True)
     oc = O.crystal(slip_set=wl.material.slip_set, jac_exact=wl.material.jac_exact)
+    os.environ["ORC_PROGRESS"] = "1"  # one line per Newton iteration
     t = time.perf_counter()
     Po, co, rco, _ = O.fft_nr3(oc, wl.N, ang, wl.dt, BC[:nstep], isNBC, wl.tolPCG, wl.tolNR,
                                wl.maxIter, gamma_mode=wl.gamma_mode)
@@ -185,23 +193,31 @@ def timed_steps(s, wl, BC_all, isNBC, W, K):
     return t1 - t0, rc, Pbar, cnt, times, [b - a for a, b in zip(c0, c1)], avg, nlaunch
 
 
-def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
-    """The same W + K steps with the learned warm start (trained on pairs
-    logged during the W warm-up steps)."""
+def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain, mode):
+    """The same W + K steps with the warm start of the local solves on for
+    the K timed steps (cpfft_set_warm_start mode).  With the network source
+    the network is trained on pairs logged during the W warm-up steps."""
     V = wl.N ** 3
     s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
     s.autotune()  # the shapes chosen for the plain run (cached per N)
-    stride = max(1, (W * V) // (1 << 21))  # at most ~2M logged pairs
-    s.mlp_init(seed=1, capacity=W * ((V + stride - 1) // stride), batch=4096)
-    rc, _, _, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1,
-                            mlp_log_last=W, mlp_log_stride=stride)
-    if rc != 0:
-        raise SystemExit(f"warm-start warm-up failed with code {rc}")
-    t = time.perf_counter()
-    iters = 3000
-    loss = s.mlp_train(iters, lr=2e-3)
-    t_train = time.perf_counter() - t
-    s.mlp_warm_start(True)
+    training = None
+    if mode & C.WARM_NETWORK:
+        stride = max(1, (W * V) // (1 << 21))  # at most ~2M logged pairs
+        s.mlp_init(seed=1, capacity=W * ((V + stride - 1) // stride), batch=4096)
+        rc, _, _, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1,
+                                mlp_log_last=W, mlp_log_stride=stride)
+        if rc != 0:
+            raise SystemExit(f"warm-start warm-up failed with code {rc}")
+        t = time.perf_counter()
+        iters = 3000
+        loss = s.mlp_train(iters, lr=2e-3)
+        training = {"pairs": int(s.mlp_stats()[0]), "log_stride": stride, "adam_steps": iters,
+                    "seconds": time.perf_counter() - t, "final_mse_normalised": loss}
+    else:
+        rc, _, _, _ = s.compute(BC_all[:W], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter, first_step=1)
+        if rc != 0:
+            raise SystemExit(f"warm-start warm-up failed with code {rc}")
+    s.set_warm_start(mode)
     wall, rc, Pbar, cnt, times, d, avg, _ = timed_steps(s, wl, BC_all, isNBC, W, K)
     if rc != 0:
         raise SystemExit(f"warm-started steps failed with code {rc}")
@@ -210,7 +226,9 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
     n_des = d[2]
     lw = (d[3] + d[4]) / max(1, n_des * V)
     lp = (d_plain[3] + d_plain[4]) / max(1, d_plain[2] * V)
-    return {
+    out = {
+        "mode": {C.WARM_NETWORK: "network", C.WARM_PREVIOUS: "previous iterate",
+                 C.WARM_NETWORK | C.WARM_PREVIOUS: "network and previous iterate"}[mode],
         "value": K / wall, "unit": "load steps/s", "ms_per_step": 1e3 * wall / K,
         "voxel_updates_per_s": V * n_des / wall,
         "drive_eps_sig_ms": avg[1],
@@ -218,10 +236,11 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain):
         "plain_local_iterations_per_voxel_update": lp,
         "Pbar_max_rel_diff_vs_plain": float(abs(Pbar - Pbar_plain).max() / abs(Pbar_plain).max()),
         "newton_per_step": float(cnt[:, 0].mean()),
-        "training": {"pairs": int(st[0]), "log_stride": stride, "adam_steps": iters,
-                     "seconds": t_train, "final_mse_normalised": loss},
-        "fallbacks_to_plain_start": int(st[3]),
+        "redone_from_plain_start": int(st[3]),
     }
+    if training:
+        out["training"] = training
+    return out
 
 
 def batch_bench(name, W, K, nthreads):
@@ -415,9 +434,15 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, s, (n_gkdf / K, n_des / K))
     s.close()
-    if not args.no_warm_start and wl.material.slip_set == 1:
-        print("[bench] warm-start leg", flush=True)
-        out["warm_start"] = warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar, d)
+    out["per_step"]["drive_eps_sig_ms"] = avg[1]
+    if not args.no_warm_start:
+        modes = [C.WARM_PREVIOUS]
+        if wl.material.slip_set in (1, 80):  # the network serves fcc and bcc24
+            modes.append(C.WARM_NETWORK | C.WARM_PREVIOUS)
+        out["warm_start"] = []
+        for mode in modes:
+            print(f"[bench] warm-start leg, mode {mode}", flush=True)
+            out["warm_start"].append(warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar, d, mode))
     print(json.dumps(out), flush=True)
 
 

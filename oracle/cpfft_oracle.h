@@ -99,6 +99,8 @@ void orc_ddot42n(long V, const double* A4, const double* B2, double* C2);
 
 /* ---- solver pieces (FFT_nr3.f) ---------------------------------------- */
 /* fftPcg (FFT_nr3.f:214-360); returns 0 ok, 1 zero RHS, -1 bad tol, -2 maxit */
+int  orc_pcg_limit(orc_fft* f, const double* K4, const double* b, double* x, double tol,
+                    int maxIter, int* iters, double* relres);
 int  orc_pcg(orc_fft* f, const double* K4, const double* b, double* x, double tol,
              int* iters, double* relres);
 int  orc_tangent_homo(orc_fft* f, const double* K4, double tol, double C_homo[81]);

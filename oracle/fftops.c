@@ -237,8 +237,14 @@ static double dot(long n, const double* x, const double* y) {
  * stop when ||r|| <= tol*||b|| or ||r|| <= tol, at most 1000 iterations */
 int orc_pcg(orc_fft* f, const double* K4, const double* b, double* x, double tol, int* iters,
             double* relres) {
+  return orc_pcg_limit(f, K4, b, x, tol, 1000, iters, relres);
+}
+
+/* fftPcg with an iteration limit other than the reference's 1000 (a timing
+ * sample runs a fixed number of iterations) */
+int orc_pcg_limit(orc_fft* f, const double* K4, const double* b, double* x, double tol,
+                  int maxIter, int* iters, double* relres) {
   long n = 9 * f->V;
-  const int maxIter = 1000;
   for (long i = 0; i < n; ++i) x[i] = 0.0;
   *iters = 0;
   *relres = 0.0;
@@ -290,6 +296,10 @@ int orc_tangent_homo_it(orc_fft* f, const double* K4, double tol, double C_homo[
     int st = orc_pcg(f, K4, b, C, tol, &it, &rel);
     if (st < 0 && status == 0) status = st;
     if (iters) *iters += it;
+    if (getenv("ORC_PROGRESS")) {
+      fprintf(stderr, "oracle: tangent_homo column %d (%d CG)\n", i + 1, it);
+      fflush(stderr);
+    }
     for (long e = 0; e < V; ++e) C[i * V + e] += 1.0;
     for (int j = 0; j < 9; ++j) {
       double s = dot(n, K4 + (long)9 * j * V, C);
@@ -351,9 +361,12 @@ static void axpy(long n, double a, const double* x, double* y) {
 }
 
 /* FFT_nr3 (FFT_nr3.f:14-200), preceded by the elastic call of `compute`
- * (FFT_finite_3d.f:145: drive_eps_sig(1,0)) */
+ * (FFT_finite_3d.f:145: drive_eps_sig(1,0)).  With the environment variable
+ * ORC_PROGRESS set, each Newton iteration prints a line to stderr (the
+ * reference's verbose residual line, FFT_nr3.f:114). */
 int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNBC[9],
                 double* Pbar_out, int* counters_out) {
+  const int progress = getenv("ORC_PROGRESS") != NULL;
   const orc_crystal* cp = pb->cp;
   long V = pb->V, n = 9 * V;
   int W = ORC_NWORDS(cp->nslip);
@@ -407,6 +420,11 @@ int orc_fft_nr3(orc_problem* pb, int nstep, const double* BC_all, const int isNB
         cgits += it;
         axpy(n, 1.0, dFm, pb->Fn1);
         resfft = nrm2(n, dFm) / Fnorm;
+        if (progress) {
+          fprintf(stderr, "oracle: step %d mixed-BC loop %d Newton %d |dF|/|F| = %.4e (%d CG)\n",
+                  step, iiter_NBC, iiter_EBC, resfft, it);
+          fflush(stderr);
+        }
         if (iiter_EBC == pb->maxIter) { rc = -4; goto done; }
         iiter_EBC++;
         newton++;

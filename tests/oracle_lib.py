@@ -96,6 +96,8 @@ def lib():
         L.orc_gkdf.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int]
         L.orc_ddot42n.argtypes = [ctypes.c_long, _dp, _dp, _dp]
         L.orc_pcg.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _ip, _dp]
+        L.orc_pcg_limit.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                    _ip, _dp]
         L.orc_tangent_homo.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
         L.orc_nbc_update.argtypes = [_dp, _dp, _dp, _dp, _ip]
         L.orc_fft_nr3.argtypes = [ctypes.POINTER(OrcProblem), ctypes.c_int, _dp, _ip, _dp,
@@ -184,12 +186,17 @@ class FFT:
                            P(out), 1)
         return out
 
-    def pcg(self, K4, b, tol):
+    def pcg(self, K4, b, tol, maxit=None):
+        """fftPcg; maxit bounds the iterations (the reference's 1000 if None)"""
         x = np.zeros((9, self.V))
         it = ctypes.c_int(0)
         rel = ctypes.c_double(0)
-        rc = lib().orc_pcg(self.h, P(np.ascontiguousarray(K4)), P(np.ascontiguousarray(b)),
-                           P(x), tol, ctypes.byref(it), ctypes.byref(rel))
+        K4c, bc = np.ascontiguousarray(K4), np.ascontiguousarray(b)
+        if maxit is None:
+            rc = lib().orc_pcg(self.h, P(K4c), P(bc), P(x), tol, ctypes.byref(it), ctypes.byref(rel))
+        else:
+            rc = lib().orc_pcg_limit(self.h, P(K4c), P(bc), P(x), tol, int(maxit), ctypes.byref(it),
+                                     ctypes.byref(rel))
         return x, rc, it.value, rel.value
 
     def tangent_homo(self, K4, tol):
