@@ -218,7 +218,9 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain, mode):
         if rc != 0:
             raise SystemExit(f"warm-start warm-up failed with code {rc}")
     s.set_warm_start(mode)
+    l0 = s.local_stats()
     wall, rc, Pbar, cnt, times, d, avg, _ = timed_steps(s, wl, BC_all, isNBC, W, K)
+    l1 = s.local_stats()
     if rc != 0:
         raise SystemExit(f"warm-started steps failed with code {rc}")
     st = s.mlp_stats()
@@ -234,6 +236,7 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain, mode):
         "drive_eps_sig_ms": avg[1],
         "local_iterations_per_voxel_update": lw,
         "plain_local_iterations_per_voxel_update": lp,
+        "local_iteration_lane_efficiency": (l1[0] - l0[0]) / max(1, l1[1] - l0[1]),
         "Pbar_max_rel_diff_vs_plain": float(abs(Pbar - Pbar_plain).max() / abs(Pbar_plain).max()),
         "newton_per_step": float(cnt[:, 0].mean()),
         "redone_from_plain_start": int(st[3]),
@@ -367,7 +370,9 @@ def main():
     if rc != 0:
         raise SystemExit(f"warm-up failed with code {rc}")
     print(f"[bench] warm-up done: counters {cw.tolist()}", flush=True)
+    l0 = s.local_stats()
     wall, rc, Pbar, cnt, times, d, avg, nlaunch = timed_steps(s, wl, BC_all, isNBC, W, K)
+    l1 = s.local_stats()
     if rc != 0:
         raise SystemExit(f"timed steps failed with code {rc}")
     V = wl.N ** 3
@@ -402,6 +407,9 @@ def main():
             "newton": float(cnt[:, 0].mean()), "mixed_bc_loops": float(cnt[:, 2].mean()),
             "local_newton_iters_per_voxel_call": d[4] / max(1, n_des * V),
             "local_failures": d[6],
+            # local iterations of the voxels / iterations their wavefronts
+            # issue (64 x the slowest voxel of each wavefront)
+            "local_iteration_lane_efficiency": (l1[0] - l0[0]) / max(1, l1[1] - l0[1]),
         },
         "category_s": {"pcg": float(times[0]), "sig_eps": float(times[1]), "total": float(times[2])},
         "kernel_ms": {"cg_iteration_group": avg[0], "drive_eps_sig": avg[1], "cg_check": avg[2],

@@ -48,7 +48,7 @@ EXPORTS = [
     "cpfft_kernel_times", "cpfft_set_timing", "cpfft_set_graphs", "cpfft_set_cg_maxit", "cpfft_autotune",
     "cpfft_get_tune", "cpfft_set_tune",
     "cpfft_mlp_init", "cpfft_mlp_log", "cpfft_mlp_train", "cpfft_mlp_warm_start",
-    "cpfft_set_warm_start",
+    "cpfft_set_warm_start", "cpfft_local_stats",
     "cpfft_mlp_params", "cpfft_mlp_forward", "cpfft_mlp_grad", "cpfft_mlp_predict",
     "cpfft_mlp_stats",
 ]
@@ -132,6 +132,7 @@ def lib():
         L.cpfft_mlp_train.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
         L.cpfft_mlp_warm_start.argtypes = [_vp, ctypes.c_int]
         L.cpfft_set_warm_start.argtypes = [_vp, ctypes.c_int]
+        L.cpfft_local_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong)]
         L.cpfft_mlp_params.argtypes = [_vp, _fp, ctypes.c_int]
         L.cpfft_mlp_forward.argtypes = [_vp, _fp, ctypes.c_int, _fp]
         L.cpfft_mlp_grad.argtypes = [_vp, _fp, _fp, ctypes.c_int, _fp, _dp]
@@ -389,6 +390,13 @@ class Solver:
         ws = np.zeros((nout, self.V), dtype=np.float32)
         _check(lib().cpfft_mlp_predict(self.h, int(step), _pf(X), _pf(ws)), "cpfft_mlp_predict")
         return X, ws
+
+    def local_stats(self):
+        """(voxel local iterations, issued local iterations, line-search
+        evaluations, sub-step cuts) since creation"""
+        out = (ctypes.c_longlong * 4)()
+        _check(lib().cpfft_local_stats(self.h, out), "cpfft_local_stats")
+        return list(out)
 
     def mlp_stats(self):
         out = (ctypes.c_longlong * 4)()

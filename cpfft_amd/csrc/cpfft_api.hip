@@ -832,7 +832,12 @@ extern "C" int cpfft_drive_eps_sig(cpfft_ctx* c, int step, int iiter) {
   tstart(c, 1);
   const float* ws = nullptr;
   const int wp = (c->warm & CPFFT_WARM_PREVIOUS) && iiter > 0;
-  if ((c->warm & CPFFT_WARM_NETWORK) && iiter > 0 && c->nslip <= kMlpMaxOut) {
+  // with the previous iterate on, the network serves the first Newton
+  // iteration of a mixed-BC loop only, where the previous iterate is the
+  // explicit update of iiter 0; later iterates are better starts than a
+  // prediction
+  const bool net = (c->warm & CPFFT_WARM_NETWORK) && iiter > 0 && (!wp || iiter == 1);
+  if (net && c->nslip <= kMlpMaxOut) {
     if (c->nslip == 12)
       mlp_predict<12>(c, step);
     else
@@ -1273,6 +1278,18 @@ extern "C" int cpfft_mlp_predict(cpfft_ctx* c, int step, float* X, float* ws) {
                           c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
   }
+  return 0;
+}
+
+extern "C" int cpfft_local_stats(cpfft_ctx* c, long long out[4]) {
+  if (!c) return fail(-1, "cpfft_local_stats: null context");
+  long long* d = c->hcnt;
+  HIP_OK(hipMemcpyAsync(d, c->dcnt, 8 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  out[0] = d[0] + d[1];  // local iterations of the voxels (predictor + full Newton)
+  out[1] = d[6];         // local iterations issued: per wavefront 64 x its slowest voxel's
+  out[2] = d[4];         // residual evaluations of the line searches
+  out[3] = d[2];         // sub-step cuts
   return 0;
 }
 

@@ -752,6 +752,12 @@ __global__ __launch_bounds__(64) void k_drive_eps_sig(long V, MatParams P,
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)&counters[k], (unsigned long long)v);
   }
+  // local iterations the wavefront issues: 64 lanes times its slowest lane's
+  // count (counters[6]; against counters[0] + counters[1], the lanes' own
+  // iterations, it measures the divergence of the local solves)
+  long long m = cnt.pred_it + cnt.upd_it;
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (long long)__shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)(64 * m));
 }
 
 // K4 = dP/dF from the tangent of k_drive_eps_sig (cep2A)

@@ -283,7 +283,9 @@ int cpfft_mlp_train(cpfft_ctx* ctx, int iters, double lr, uint64_t seed, double*
  * of the reference's start).  mode is a bit set of the candidate starts:
  * CPFFT_WARM_NETWORK the network's predicted slip increments (needs
  * cpfft_mlp_init; fcc and bcc24), CPFFT_WARM_PREVIOUS the step's previous
- * global iterate held in STATE_NP1.  Per voxel the candidate of smaller
+ * global iterate held in STATE_NP1.  With both, the network serves only
+ * iiter == 1 (where the previous iterate is the explicit update of iiter
+ * 0) and the previous iterate the later calls.  Per voxel the candidate of smaller
  * full residual replaces the reference's predictor stage when its residual
  * is below the predictor's starting residual; the Newton tolerances are
  * unchanged and a failed warm solve is redone from the reference's start.
@@ -308,6 +310,12 @@ int cpfft_mlp_grad(cpfft_ctx* ctx, const float* Xn, const float* Tn, int n, floa
  * normalised resolved shears the local solve starts from, [nout][V] --
  * what cpfft_drive_eps_sig uses when the warm start is on                 */
 int cpfft_mlp_predict(cpfft_ctx* ctx, int step, float* X, float* ws);
+/* Divergence of the local solves since creation (no counterpart in the
+ * reference): [0] local iterations of the voxels (predictor + full Newton),
+ * [1] local iterations issued, i.e. per wavefront of 64 voxels 64 times its
+ * slowest voxel's count, [2] line-search residual evaluations, [3] sub-step
+ * cuts.  [0] / [1] is the fraction of the issued iterations doing work.   */
+int cpfft_local_stats(cpfft_ctx* ctx, long long out[4]);
 /* [0] logged pairs, [1] training steps, [2] warm-started voxel updates,
  * [3] warm-started solves redone from the plain start                     */
 int cpfft_mlp_stats(cpfft_ctx* ctx, long long out[4]);
