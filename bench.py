@@ -246,6 +246,70 @@ def warm_start_leg(wl, ang, BC_all, isNBC, W, K, Pbar_plain, d_plain, mode):
     return out
 
 
+def online_bench(name, W, R, M):
+    """Online retraining at full size: after W plain steps, M rounds of R
+    load steps; every round logs training pairs, and before each round after
+    the first the network is retrained on the latest logged pairs (a ring of
+    one round's pairs) and warm-starts the round's local solves.  A plain run
+    of the same steps is the reference.  Reports per round the local
+    iterations, the time and the averaged-stress difference."""
+    wl = WORKLOADS[name]
+    BC, isNBC = wl.bc()
+    ang = wl.angles()
+    V = wl.N ** 3
+    args = (isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    legs = {}
+    for leg in ("plain", "online"):
+        s = C.Solver(wl.N, ang, wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+        s.autotune()
+        if leg == "online":
+            stride = max(1, (R * V) // (1 << 21))  # at most ~2M pairs a round
+            s.mlp_init(seed=1, capacity=R * ((V + stride - 1) // stride), batch=4096)
+        rc, _, _, _ = s.compute(BC[:W], *args)
+        if rc != 0:
+            raise SystemExit(f"{leg}: warm-up failed with code {rc}")
+        rounds = []
+        for k in range(M):
+            first = W + k * R + 1
+            info = {"steps": [first, first + R - 1]}
+            if leg == "online" and k > 0:
+                t = time.perf_counter()
+                info["mse_normalised"] = s.mlp_train(1000, lr=2e-3)
+                info["train_s"] = time.perf_counter() - t
+                s.set_warm_start(C.WARM_NETWORK)
+            c0 = s.counters()
+            t = time.perf_counter()
+            kw = {"mlp_log_last": first + R - 1, "mlp_log_stride": stride} if leg == "online" else {}
+            rc, P, cnt, _ = s.compute(BC[first - 1:first - 1 + R], *args, first_step=first, **kw)
+            info["wall_s"] = time.perf_counter() - t
+            if rc != 0:
+                raise SystemExit(f"{leg}: round {k} failed with code {rc}")
+            c1 = s.counters()
+            info["local_iterations_per_voxel_update"] = (
+                (c1[3] - c0[3]) + (c1[4] - c0[4])) / max(1, (c1[2] - c0[2]) * V)
+            info["newton"] = cnt[:, 0].tolist()
+            info["P_xx"] = P[:, 0].tolist()
+            rounds.append((info, P))
+            print(f"[online] {leg} round {k}: {json.dumps(info)}", flush=True)
+        legs[leg] = rounds
+        if leg == "online":
+            legs["stats"] = s.mlp_stats()
+        s.close()
+    out = {"workload": f"{name}: {wl.N}^3, {wl.ngrains} grains, {workload_text(wl)}, dt={wl.dt}s",
+           "warmup_steps": W, "round_steps": R, "rounds": M, "network_pairs_per_round": None,
+           "rounds_plain": [i for i, _ in legs["plain"]],
+           "rounds_online": [i for i, _ in legs["online"]]}
+    st = legs["stats"]
+    out["network_pairs_held"] = int(st[0])
+    out["warm_started_voxel_updates"] = int(st[2])
+    out["redone_from_plain_start"] = int(st[3])
+    out["Pbar_max_rel_diff_vs_plain"] = max(
+        float(abs(Po - Pp).max() / abs(Pp).max())
+        for (_, Pp), (_, Po) in zip(legs["plain"], legs["online"]))
+    del out["network_pairs_per_round"]
+    return out
+
+
 def batch_bench(name, W, K, nthreads):
     """Batched uneven RVEs: the members' load steps W+1..W+K timed twice
     from the same warm-up state, once one RVE after another on one host
@@ -337,6 +401,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--threads", type=int, default=8, help="host threads of a batched workload")
+    ap.add_argument("--online", type=int, nargs=2, default=None, metavar=("R", "M"),
+                    help="online retraining run of --workload: M rounds of R load steps "
+                         "after --warmup plain steps, then exit")
     ap.add_argument("--validate-cpu-baseline", type=int, default=0, metavar="NSTEP",
                     help="time a whole oracle run of NSTEP load steps of --workload against "
                          "the extrapolated CPU baseline, then exit")
@@ -348,6 +415,9 @@ def main():
     if args.validate_cpu_baseline > 0:
         print(json.dumps(validate_cpu_baseline(args.workload, args.validate_cpu_baseline)),
               flush=True)
+        return
+    if args.online:
+        print(json.dumps(online_bench(args.workload, args.warmup, *args.online)), flush=True)
         return
     if args.workload in BATCH_WORKLOADS:
         print(json.dumps(batch_bench(args.workload, args.warmup, args.steps, args.threads)),
