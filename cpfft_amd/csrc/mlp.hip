@@ -278,15 +278,28 @@ void mlp_stats(const MlpBuffers& b, long count, hipStream_t s) {
 // One launch over all voxels.  The block stages the BF16 weights (rows
 // padded by eight elements against LDS bank conflicts) and the FP32 biases
 // and output normalisation in LDS; each wavefront then takes 16 voxels at a
-// time through the three layers with v_mfma_f32_16x16x32_bf16 (lane l
-// supplies A row l%16 and B column l%16 at k = 8*(l/16)..+7 of each K step
-// of 32, and holds C rows 4*(l/16)..+3 of column l%16).  Layer outputs go
-// through a per-wavefront LDS tile (16 x 128 BF16) that the next layer reads
-// as its A operand.  The rounding points are those of mlp_forward: FP32
-// bias and tanh, BF16 hidden activations, FP32 output.
+// time through the three layers with v_mfma_f32_16x16x32_bf16, the weights
+// as the A operand (rows = output units) and the activations as the B
+// operand (columns = the 16 voxels).  Lane l supplies A row l%16 and B
+// column l%16 at k = 8*(l/16)..+7 of each K step of 32, and holds C rows
+// 4*(l/16)..+3 of column l%16.  So after a layer, lane l holds units
+// 16t + 4g + j (g = l/16, j < 4) of tile t for voxel l%16, and the eight
+// values of tiles 2s and 2s+1 are that lane's B operand of the next layer's
+// K step s, once the K index of the next weights is permuted the same way:
+// position 8g + 4h + j of K step s is unit 32s + 16h + 4g + j.  The
+// weights are stored permuted in LDS, the activations never leave the
+// registers.  The rounding points are those of mlp_forward: FP32 bias and
+// tanh, BF16 hidden activations, FP32 output.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 static constexpr int kIW = 8;               // wavefronts per inference block
-static constexpr int kHS = kMlpHid + 8;     // padded row of a hidden tile / W2, W3 row
+static constexpr int kHS = kMlpHid + 8;     // padded row of W2, W3
+
+// the unit at K position p (< 128) of the permuted order
+__device__ __forceinline__ int perm_k(int p) {
+  const int s = p >> 5, q = p & 31, g = q >> 3, h = (q >> 2) & 1, j = q & 3;
+  return 32 * s + 16 * h + 4 * g + j;
+}
 
 template <int NINP, int NOUTP>
 __global__ __launch_bounds__(64 * kIW) void k_mlp_infer(long V, int nin, int nout,
@@ -297,9 +310,8 @@ __global__ __launch_bounds__(64 * kIW) void k_mlp_infer(long V, int nin, int nou
                                                         float* __restrict__ ws) {
   constexpr int S1 = NINP + 8;
   __shared__ __align__(16) uint16_t W1[kMlpHid * S1];
-  __shared__ __align__(16) uint16_t W2[kMlpHid * kHS];
-  __shared__ __align__(16) uint16_t W3[NOUTP * kHS];
-  __shared__ __align__(16) uint16_t H[kIW][2][16 * kHS];
+  __shared__ __align__(16) uint16_t W2[kMlpHid * kHS];   // K permuted
+  __shared__ __align__(16) uint16_t W3[NOUTP * kHS];     // K permuted
   __shared__ float b1[kMlpHid], b2[kMlpHid], b3[NOUTP], mt[NOUTP], st[NOUTP];
   const int T = 64 * kIW;
   for (int i = threadIdx.x; i < kMlpHid * S1; i += T) {
@@ -308,11 +320,11 @@ __global__ __launch_bounds__(64 * kIW) void k_mlp_infer(long V, int nin, int nou
   }
   for (int i = threadIdx.x; i < kMlpHid * kHS; i += T) {
     const int n = i / kHS, k = i - n * kHS;
-    W2[i] = k < kMlpHid ? pbf[d.offW2 + (long)n * kMlpHid + k] : 0;
+    W2[i] = k < kMlpHid ? pbf[d.offW2 + (long)n * kMlpHid + perm_k(k)] : 0;
   }
   for (int i = threadIdx.x; i < NOUTP * kHS; i += T) {
     const int n = i / kHS, k = i - n * kHS;
-    W3[i] = (n < nout && k < kMlpHid) ? pbf[d.offW3 + (long)n * kMlpHid + k] : 0;
+    W3[i] = (n < nout && k < kMlpHid) ? pbf[d.offW3 + (long)n * kMlpHid + perm_k(k)] : 0;
   }
   for (int i = threadIdx.x; i < kMlpHid; i += T) {
     b1[i] = params[d.offB1 + i];
@@ -326,91 +338,91 @@ __global__ __launch_bounds__(64 * kIW) void k_mlp_infer(long V, int nin, int nou
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, kg = lane >> 4;
-  uint16_t* H1 = H[w][0];
-  uint16_t* H2 = H[w][1];
-  // per-wavefront exchange through LDS: every lane's writes complete before
-  // any lane reads
-  auto wave_sync = [] {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  };
   const long nrb = (V + 15) / 16;
   for (long rb = (long)blockIdx.x * kIW + w; rb < nrb; rb += (long)gridDim.x * kIW) {
-    const long r0 = rb * 16;
-    // layer 1: A = the 16 voxels' inputs straight from HBM
-    bf16x8 a1[NINP / 32];
-    {
-      const long r = r0 + li;
+    const long r = rb * 16 + li;  // this lane's voxel (B column)
+    // layer 1: B = the voxel's inputs straight from HBM
+    bf16x8 x1[NINP / 32];
 #pragma unroll
-      for (int ks = 0; ks < NINP / 32; ++ks) {
-        if (r < V) {
-          a1[ks] = *reinterpret_cast<const bf16x8*>(Xv + r * NINP + ks * 32 + kg * 8);
-        } else {
+    for (int ks = 0; ks < NINP / 32; ++ks) {
+      if (r < V) {
+        x1[ks] = *reinterpret_cast<const bf16x8*>(Xv + r * NINP + ks * 32 + kg * 8);
+      } else {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) a1[ks][j] = (__bf16)0.0f;
-        }
+        for (int j = 0; j < 8; ++j) x1[ks][j] = (__bf16)0.0f;
       }
     }
+    // a layer's output as the next layer's B operand: K step s packs tiles
+    // 2s (elements 0-3) and 2s+1 (elements 4-7), two BF16 per 32-bit word
+    u32x4 h[kMlpHid / 32];
+    // bias, tanh and BF16 rounding of tile t's four values, packed in pairs
+    auto act = [&](f32x4 acc, const float* bias, int t, uint32_t* p) {
 #pragma unroll
-    for (int jt = 0; jt < kMlpHid / 16; ++jt) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int q = 0; q < 2; ++q) {
+        const float v0 = tanhf(acc[2 * q] + bias[16 * t + 4 * kg + 2 * q]);
+        const float v1 = tanhf(acc[2 * q + 1] + bias[16 * t + 4 * kg + 2 * q + 1]);
+        p[q] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+      }
+    };
+#pragma unroll
+    for (int s2 = 0; s2 < kMlpHid / 32; ++s2) {
+      f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < NINP / 32; ++ks) {
-        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(W1 + (16 * jt + li) * S1 + ks * 32 + kg * 8);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[ks], bw, acc, 0, 0, 0);
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(W1 + (32 * s2 + li) * S1 + ks * 32 + kg * 8);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(W1 + (32 * s2 + 16 + li) * S1 + ks * 32 + kg * 8);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, x1[ks], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, x1[ks], acc1, 0, 0, 0);
       }
-      const int n = 16 * jt + li;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) H1[(4 * kg + j) * kHS + n] = f2bf(tanhf(acc[j] + b1[n]));
+      uint32_t p[4];
+      act(acc0, b1, 2 * s2, p);
+      act(acc1, b1, 2 * s2 + 1, p + 2);
+      h[s2] = u32x4{p[0], p[1], p[2], p[3]};
+      // keep the scheduler from hoisting every tile's weight loads (the
+      // registers they would hold cost occupancy)
+      __builtin_amdgcn_sched_barrier(0);
     }
-    wave_sync();
     // layer 2
-    bf16x8 a2[kMlpHid / 32];
+    u32x4 h2[kMlpHid / 32];
 #pragma unroll
-    for (int ks = 0; ks < kMlpHid / 32; ++ks)
-      a2[ks] = *reinterpret_cast<const bf16x8*>(H1 + li * kHS + ks * 32 + kg * 8);
-#pragma unroll
-    for (int jt = 0; jt < kMlpHid / 16; ++jt) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s2 = 0; s2 < kMlpHid / 32; ++s2) {
+      f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < kMlpHid / 32; ++ks) {
-        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(W2 + (16 * jt + li) * kHS + ks * 32 + kg * 8);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw, acc, 0, 0, 0);
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(W2 + (32 * s2 + li) * kHS + ks * 32 + kg * 8);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(W2 + (32 * s2 + 16 + li) * kHS + ks * 32 + kg * 8);
+        const bf16x8 b = __builtin_bit_cast(bf16x8, h[ks]);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc1, 0, 0, 0);
       }
-      const int n = 16 * jt + li;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) H2[(4 * kg + j) * kHS + n] = f2bf(tanhf(acc[j] + b2[n]));
+      uint32_t p[4];
+      act(acc0, b2, 2 * s2, p);
+      act(acc1, b2, 2 * s2 + 1, p + 2);
+      h2[s2] = u32x4{p[0], p[1], p[2], p[3]};
+      __builtin_amdgcn_sched_barrier(0);
     }
-    wave_sync();
     // layer 3 and the output normalisation
-#pragma unroll
-    for (int ks = 0; ks < kMlpHid / 32; ++ks)
-      a2[ks] = *reinterpret_cast<const bf16x8*>(H2 + li * kHS + ks * 32 + kg * 8);
 #pragma unroll
     for (int jt = 0; jt < NOUTP / 16; ++jt) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < kMlpHid / 32; ++ks) {
-        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(W3 + (16 * jt + li) * kHS + ks * 32 + kg * 8);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw, acc, 0, 0, 0);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W3 + (16 * jt + li) * kHS + ks * 32 + kg * 8);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h2[ks]), acc, 0, 0, 0);
       }
-      const int n = 16 * jt + li;
-      if (n < nout) {
+      if (r < V) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const long r = r0 + 4 * kg + j;
-          if (r < V) ws[n * V + r] = (acc[j] + b3[n]) * st[n] + mt[n];
+          const int n = 16 * jt + 4 * kg + j;
+          if (n < nout) ws[n * V + r] = (acc[j] + b3[n]) * st[n] + mt[n];
         }
       }
     }
-    // the next row block rewrites H1, H2: all lanes are done reading them
-    wave_sync();
   }
 }
 
 void mlp_infer(const MlpBuffers& b, long V, hipStream_t s) {
-  const unsigned g = 256;  // one block per CU (the LDS holds one); each loops over row blocks
+  const unsigned g = 512;  // two blocks per CU (their weights fit the LDS); each loops over row blocks
   const MlpDims& d = b.d;
   if (mlp_nin_pad(d.nin) == 64 && d.nout <= 16)
     k_mlp_infer<64, 16><<<g, 64 * kIW, 0, s>>>(V, d.nin, d.nout, b.Xv, b.pbf, b.params, d, b.norm,
