@@ -48,9 +48,11 @@ TIMING_STRIDE = 7
 # forms alpha; the three middle passes apply the projection; pass D applies
 # x += alpha p and r -= alpha q without storing q and writes the r.r partial
 # sums (only the single-block convergence test is outside the group).
-# Algorithmic bytes per voxel: read K4 (648), p, r and x, write p, r and x
-# (72 each).
-CG_ITERATION_BYTES_PER_VOXEL = 648.0 + 6 * 72.0
+# Algorithmic bytes per voxel, counted per CG iteration of fftPcg as G_K_dF
+# with K4 (read the direction 72 + read K4 648 + write q 72 = 792) plus five
+# 72-byte vector streams of the r, p, x, q updates and dot products.  (The
+# fused group itself needs only 1080: read K4, p, r, x, write p, r, x.)
+CG_ITERATION_BYTES_PER_VOXEL = 792.0 + 5 * 72.0
 
 
 def cg_iteration_traffic_per_voxel(N):
