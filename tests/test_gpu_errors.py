@@ -10,6 +10,7 @@ every kernel still exits normally:
   (fftPcg's die_abort, FFT_nr3.f:248-251), also in a continued run whose
   steps would otherwise never reach a CG solve with a nonzero right-hand
   side.
+* the warm-start mode's argument checks, and the local-solve counters.
 """
 import numpy as np
 import pytest
@@ -71,4 +72,27 @@ def test_driver_stops_on_an_improper_cg_tolerance():
     Fb = np.tile(s.driver_state()[0], (1, 1))
     rc, _, _, _ = s.compute(Fb, allF, 2.0, wl.tolNR, wl.maxIter, first_step=2)
     assert rc == -1
+    s.close()
+
+
+def test_warm_start_mode_errors_and_local_stats():
+    """cpfft_set_warm_start rejects unknown bits and a network start without
+    a network; the previous-iterate start needs no network.  The local-solve
+    counters: the iterations wavefronts issue are at least the voxels' own."""
+    s = _solver(8)
+    with pytest.raises(RuntimeError):
+        s.set_warm_start(4)
+    with pytest.raises(RuntimeError):
+        s.set_warm_start(C.WARM_NETWORK)
+    s.set_warm_start(C.WARM_PREVIOUS)
+    s.set_warm_start(0)
+    l0 = s.local_stats()
+    s.fill9(C.DFM, np.array([5e-4, 0, 0, 0, -2.5e-4, 0, 0, 0, -2.5e-4]))
+    s.daxpy(1.0, C.DFM, C.FN1)
+    assert s.drive_eps_sig(1, 1) == 0
+    l1 = s.local_stats()
+    useful, issued = l1[0] - l0[0], l1[1] - l0[1]
+    print(f"\n[local] iterations {useful}, issued {issued}", flush=True)
+    assert useful > 0 and issued >= useful
+    assert issued % 64 == 0
     s.close()
