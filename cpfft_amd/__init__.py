@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(PKG, "libcpfft_amd.so")
 DRIVER = os.path.join(PKG, "libcpfft_driver.so")
 
-FN, FN1, PN, PN1, DFM, B, K4, STATE_N, STATE_NP1, WORK = range(10)
+FN, FN1, PN, PN1, DFM, B, K4, STATE_N, STATE_NP1, WORK, EPS = range(11)
 WARM_NETWORK, WARM_PREVIOUS = 1, 2  # include/cpfft.h CPFFT_WARM_*
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -208,7 +208,7 @@ class Solver:
         return lib().cpfft_field(self.h, f)
 
     def _len(self, f):
-        return {K4: 81, STATE_N: self.nwords, STATE_NP1: self.nwords}.get(f, 9) * self.V
+        return {K4: 81, STATE_N: self.nwords, STATE_NP1: self.nwords, EPS: 6}.get(f, 9) * self.V
 
     def upload(self, f, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).ravel()

@@ -232,7 +232,8 @@ extern "C" int cpfft_create(int N, const cpfft_material* m, const double* angles
   HIP_OK(hipMemcpy(c->oid, hoid.data(), V * sizeof(int32_t), hipMemcpyHostToDevice));
   // fields
   const int W = CPFFT_NWORDS(c->nslip);
-  long lens[CPFFT_NFIELDS] = {9 * V, 9 * V, 9 * V, 9 * V, 9 * V, 9 * V, 81 * V, W * V, W * V, 9 * V};
+  long lens[CPFFT_NFIELDS] = {9 * V, 9 * V, 9 * V, 9 * V, 9 * V, 9 * V, 81 * V, W * V, W * V, 9 * V,
+                              6 * V};
   for (int f = 0; f < CPFFT_NFIELDS; ++f) {
     c->field_len[f] = lens[f];
     HIP_OK(hipMalloc(&c->fields[f], lens[f] * sizeof(double)));
@@ -332,7 +333,7 @@ extern "C" int cpfft_reset(cpfft_ctx* c) {
   const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   launch_fill9(V, id, c->fields[CPFFT_FN], c->stream);
   launch_fill9(V, id, c->fields[CPFFT_FN1], c->stream);
-  for (int f : {CPFFT_PN, CPFFT_PN1, CPFFT_DFM, CPFFT_B, CPFFT_WORK, CPFFT_K4})
+  for (int f : {CPFFT_PN, CPFFT_PN1, CPFFT_DFM, CPFFT_B, CPFFT_WORK, CPFFT_K4, CPFFT_EPS})
     launch_fill(c->field_len[f], 0.0, c->fields[f], c->stream);
   // tau_tilde starts at tau_y + 1e-5 (mm10_init_voche, mm10_a.f:2042-2055)
   for (int f : {CPFFT_STATE_N, CPFFT_STATE_NP1})
@@ -1024,9 +1025,15 @@ extern "C" int cpfft_tangent_homo(cpfft_ctx* c, double tol, double C_homo[81]) {
 }
 
 extern "C" int cpfft_update(cpfft_ctx* c) {
+  if (!c) return fail(-1, "cpfft_update: null context");
   std::swap(c->fields[CPFFT_STATE_N], c->fields[CPFFT_STATE_NP1]);
   launch_copy(9 * c->V, c->fields[CPFFT_FN1], c->fields[CPFFT_FN], c->stream);
   launch_copy(9 * c->V, c->fields[CPFFT_PN1], c->fields[CPFFT_PN], c->stream);
+  // eps_n1 = eps_n + uddt of the step's last update (rstgp1.f:1388-1393),
+  // then eps_n <- eps_n1 (update.f:85); the words CPFFT_W_D..+5 are
+  // contiguous [6][V]
+  launch_axpy(6 * c->V, 1.0, c->fields[CPFFT_STATE_N] + (long)CPFFT_W_D * c->V,
+              c->fields[CPFFT_EPS], c->stream);
   HIP_OK(hipGetLastError());
   return 0;
 }

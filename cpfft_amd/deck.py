@@ -6,7 +6,7 @@ load-step driver take (per-voxel Kocks angles, cpfft_material, dt, BC_all,
 isNBC, tolerances), so a deck that drives CPFFT drives this solver.
 
 Commands follow the reference's input driver (FFT_finite_3d.f:14-160):
-  project <name>                           (ignored)
+  project [name] <name>                    structure name, 8 characters (:64-78)
   number of grid <N>                       (FFT_finite_3d.f:82-93)
   crystal <k> / properties ...             incrystal.f:13-1011
   material <name> / properties cp ...      inmat.f:154-313 (mm10 only)
@@ -17,7 +17,8 @@ Commands follow the reference's input driver (FFT_finite_3d.f:14-160):
   loading <name> / step <list> constraints c ...       inlod.f:11-105
   nonlinear analysis parameters / maximum iterations n,
       convergence tolerance NR a CG b, time step dt    indypm.f:14-68
-  output ... (ignored), compute, stop
+  output results steps <list>              load steps with result files (oudriv.f:85-160)
+  output model / data ... (ignored), compute, stop
 
 Scanner rules kept from the reference (scan.f, incrystal.f:987-994): a line
 whose first token is `c` is a comment; a trailing comma continues a
@@ -100,6 +101,8 @@ class Deck:
     dt: float = None
     computed: bool = False
     directory: str = "."
+    stname: str = ""                               # structure name (project)
+    out_steps: list = field(default_factory=list)  # steps of `output results`
 
     # ----- what the solver takes
     @property
@@ -348,7 +351,25 @@ def read_deck(path):
     toks = next_card()
     while toks is not None:
         cmd = toks[0].lower()
-        if cmd == "project" or cmd == "blocking" or cmd == "output":
+        if cmd == "project":
+            rest = toks[1:]
+            if rest and rest[0].lower() == "name":
+                rest = rest[1:]
+            if not rest:
+                raise DeckError("project: a name must follow")
+            d.stname = rest[0].strip("'\"")[:8]
+            toks = next_card()
+        elif cmd == "output":
+            rest = [t.lower() for t in toks[1:]]
+            if rest and rest[0].startswith("result"):
+                if len(rest) < 2 or not rest[1].startswith("step"):
+                    raise DeckError("output results: steps <list> must follow")
+                steps, _ = _int_list(toks[3:])
+                if not steps or min(steps) <= 0:
+                    raise DeckError("output results: a list of positive steps")
+                d.out_steps = sorted(set(steps))  # the last `output results` wins (oudriv.f:129)
+            toks = next_card()
+        elif cmd == "blocking":
             toks = next_card()
         elif cmd == "number":
             rest = [t for t in toks[1:] if t.lower() not in ("of", "grid")]

@@ -8,7 +8,7 @@ module cpfft_bindings
 
   integer(c_int), parameter :: CPFFT_FN = 0, CPFFT_FN1 = 1, CPFFT_PN = 2, &
        CPFFT_PN1 = 3, CPFFT_DFM = 4, CPFFT_B = 5, CPFFT_K4 = 6, &
-       CPFFT_STATE_N = 7, CPFFT_STATE_NP1 = 8, CPFFT_WORK = 9
+       CPFFT_STATE_N = 7, CPFFT_STATE_NP1 = 8, CPFFT_WORK = 9, CPFFT_EPS = 10
 
   interface
      function cpfft_field(ctx, f) bind(C, name="cpfft_field") result(p)

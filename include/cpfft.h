@@ -65,11 +65,16 @@ enum {
                                 removed; identical to the reference for odd N        */
 };
 
-/* device fields owned by the context (module fft globals, mod_fft.f:51-53) */
+/* device fields owned by the context (module fft globals, mod_fft.f:51-53);
+ * CPFFT_EPS [6][V] is the element strain of the reference's element blocks
+ * (eps_n, mod_eleblocks.f:68): the sum over the rolled-over load steps of
+ * the unrotated strain increment of each step's last update (state words
+ * CPFFT_W_D), accumulated by cpfft_update (rstgp1.f:1378-1397, update.f:85)
+ * and written by the result files (ouresult.f:230-243)                    */
 enum {
   CPFFT_FN = 0, CPFFT_FN1 = 1, CPFFT_PN = 2, CPFFT_PN1 = 3, CPFFT_DFM = 4, CPFFT_B = 5,
-  CPFFT_K4 = 6, CPFFT_STATE_N = 7, CPFFT_STATE_NP1 = 8, CPFFT_WORK = 9,
-  CPFFT_NFIELDS = 10
+  CPFFT_K4 = 6, CPFFT_STATE_N = 7, CPFFT_STATE_NP1 = 8, CPFFT_WORK = 9, CPFFT_EPS = 10,
+  CPFFT_NFIELDS = 11
 };
 
 /* material state words (history words of mm10, mm10_d.f:174-254) */
@@ -137,7 +142,8 @@ int cpfft_drive_eps_sig(cpfft_ctx* ctx, int step, int iiter);
 int cpfft_tangent_homo(cpfft_ctx* ctx, double tol, double C_homo[81]);
 
 /* update (update.f:16-193) together with the roll-over of FFT_nr3.f:173-175:
- * state n <- n+1, Fn <- Fn1, Pn <- Pn1.                                   */
+ * state n <- n+1, Fn <- Fn1, Pn <- Pn1, and CPFFT_EPS += the new state n's
+ * strain increment (eps_n <- eps_n1 = eps_n + uddt).                      */
 int cpfft_update(cpfft_ctx* ctx);
 
 /* ---- BLAS-1 on device vectors of 9*V doubles (FFT_nr3.f:86-114) ------ */
