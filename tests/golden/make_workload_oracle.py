@@ -12,10 +12,13 @@ tests check full-size runs without minutes of CPU work at test time:
                      count, loading direction and strain rate, every
                      component of F prescribed): per member its index in
                      the batch, P per step and the counts per step
+  c4_n48_n96_oracle.npz  the first load step of the first 48^3 and the
+                     first 96^3 member (the radix-3 transforms), with P
+                     and the material state of every 256th voxel
 
 The oracle is oracle/ (the CPU restatement of FFT_nr3 and the mm10
 update).  Run from the repository root after building the oracle:
-    python tests/golden/make_workload_oracle.py [c2|c4]
+    python tests/golden/make_workload_oracle.py [c2|c4|c4big]
 """
 import os
 import sys
@@ -64,6 +67,22 @@ def c4():
                         Pbar=np.array(Ps), counters=np.array(cs))
 
 
+def c4big():
+    members = BATCH_WORKLOADS["c4_batch"].members()
+    keep = [next(k for k, m in enumerate(members) if m.N == n) for n in (48, 96)]
+    out = {"members": np.array(keep)}
+    for k in keep:
+        wl = members[k]
+        _, _, _, P, cnt, st = run(wl, nstep=1)
+        idx = np.arange(0, wl.N ** 3, 256)
+        tag = f"n{wl.N}"
+        out.update({f"{tag}_Pbar": P, f"{tag}_counters": cnt, f"{tag}_idx": idx,
+                    f"{tag}_P": st["Pn"][:, idx], f"{tag}_state": st["sn"][:, idx]})
+        print(f"c4 member {k} (N={wl.N}): P_bar_xx {P[:, 0].tolist()} counters {cnt.tolist()}",
+              flush=True)
+    np.savez_compressed(os.path.join(HERE, "c4_n48_n96_oracle.npz"), **out)
+
+
 if __name__ == "__main__":
-    for w in sys.argv[1:] or ["c2", "c4"]:
-        {"c2": c2, "c4": c4}[w]()
+    for w in sys.argv[1:] or ["c2", "c4", "c4big"]:
+        {"c2": c2, "c4": c4, "c4big": c4big}[w]()

@@ -29,3 +29,25 @@ def test_c1_16_fixture():
                               wl.maxIter, gamma_mode=wl.gamma_mode)
     assert rc == 0
     assert np.array_equal(P2, Pb[:2]) and np.array_equal(c2, g["counters"][:2])
+
+
+def test_workload_fixtures_name_their_workloads():
+    """The larger workloads' fixtures (make_workload_oracle.py; recomputing
+    them takes minutes of CPU, so only their inputs are checked here): the
+    64^3 loading, and the batch members they hold."""
+    from cpfft_amd.configs import BATCH_WORKLOADS
+    g = np.load(os.path.join(GOLDEN, "c2_64_oracle.npz"))
+    wl = WORKLOADS["c2_64"]
+    BC, isNBC = wl.bc()
+    assert np.array_equal(g["BC"], BC[:2]) and np.array_equal(g["isNBC"], isNBC)
+    assert g["Pbar"].shape == (2, 9) and g["state"].shape[1] == g["idx"].size
+    members = BATCH_WORKLOADS["c4_batch"].members()
+    g = np.load(os.path.join(GOLDEN, "c4_n32_oracle.npz"))
+    assert [k for k, m in enumerate(members) if m.N == 32] == g["members"].tolist()
+    assert g["Pbar"].shape == (len(g["members"]), 2, 9)
+    g = np.load(os.path.join(GOLDEN, "c4_n48_n96_oracle.npz"))
+    assert [members[k].N for k in g["members"]] == [48, 96]
+    for n in (48, 96):
+        assert g[f"n{n}_Pbar"].shape == (1, 9)
+        assert g[f"n{n}_idx"].tolist() == list(range(0, n ** 3, 256))
+        assert g[f"n{n}_state"].shape == (26 + 2 * 12, g[f"n{n}_idx"].size)

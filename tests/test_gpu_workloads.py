@@ -12,7 +12,9 @@
   oracle on a slab of 16384 voxels of the real state.
 * the batch of 24 RVEs (c4_batch): two load steps of all 24 in one
   batched run, every 32^3 member against the oracle's run of it
-  (tests/golden/c4_n32_oracle.npz).
+  (tests/golden/c4_n32_oracle.npz); and the first load step of its first
+  48^3 and 96^3 members (radix-3 transforms in the whole driver) against
+  tests/golden/c4_n48_n96_oracle.npz: P-bar, the counts, sampled voxels.
 """
 import os
 
@@ -152,3 +154,35 @@ def test_c4_batch_first_steps_against_oracle_fixture():
         assert np.array_equal(cnt, g["counters"][j]), k
     for sv in solvers:
         sv.close()
+
+
+def _check_state_blocks(sg, so, ns, what):
+    blocks = {"stress": (0, 6), "Rp": (6, 15), "tt": (15, 17), "D": (17, 23),
+              "slip": (23, 23 + 2 * ns), "work": (23 + 2 * ns, 26 + 2 * ns)}
+    err = {k: rel(sg[a:b], so[a:b]) for k, (a, b) in blocks.items()}
+    print(f"[{what}] state rel diffs " + ", ".join(f"{k} {v:.1e}" for k, v in err.items()),
+          flush=True)
+    for k in ("stress", "Rp", "tt"):
+        assert err[k] < 1e-9, k
+    for k in ("D", "slip", "work"):  # rounding amplified, see the 64^3 test
+        assert err[k] < 1e-8, k
+
+
+def test_c4_n48_n96_members_first_step_against_oracle_fixture():
+    g = np.load(os.path.join(GOLDEN, "c4_n48_n96_oracle.npz"))
+    members = BATCH_WORKLOADS["c4_batch"].members()
+    for k in g["members"]:
+        wl = members[k]
+        tag = f"n{wl.N}"
+        BC, isNBC = wl.bc()
+        s = C.Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+        rc, Pbar, cnt, _ = s.compute(BC[:1], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+        print(f"\n[c4 member {k}, N={wl.N}] P_xx {Pbar[:, 0].tolist()} counters {cnt.tolist()}",
+              flush=True)
+        assert rc == 0
+        assert rel(Pbar, g[tag + "_Pbar"]) < 1e-9
+        assert np.array_equal(cnt, g[tag + "_counters"])
+        idx = g[tag + "_idx"]
+        assert rel(s.download(C.PN)[:, idx], g[tag + "_P"]) < 1e-9
+        _check_state_blocks(s.download(C.STATE_N)[:, idx], g[tag + "_state"], s.nslip, tag)
+        s.close()
