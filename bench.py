@@ -166,7 +166,8 @@ def validate_cpu_baseline(name, nstep):
 def workload_text(wl):
     """Crystal and loading of a workload, e.g. 'FCC cubic Voce n=20, uniaxial
     tension 1e-3/s'."""
-    crystal = {1: "FCC", 8: "BCC48", 80: "BCC24"}[wl.material.slip_set]
+    crystal = {1: "FCC", 2: "BCC", 6: "FCC (roters order)", 7: "BCC12", 8: "BCC48",
+               80: "BCC24"}[wl.material.slip_set]
     span = wl.strain * (4 * wl.cycles if wl.cycles > 0 else 1)  # strain travelled
     rate = f"{span / (wl.nstep * wl.dt):.0e}".replace("e-0", "e-")
     load = (f"uniaxial tension-compression +-{100 * wl.strain:g}% x{wl.cycles}, {rate}/s"
@@ -517,7 +518,7 @@ def main():
     out["per_step"]["drive_eps_sig_ms"] = avg[1]
     if not args.no_warm_start:
         modes = [C.WARM_PREVIOUS]
-        if wl.material.slip_set in (1, 80):  # the network serves fcc and bcc24
+        if wl.material.slip_set != 8:  # the network serves up to 24 slip systems
             modes.append(C.WARM_NETWORK | C.WARM_PREVIOUS)
         out["warm_start"] = []
         for mode in modes:

@@ -32,7 +32,8 @@ NU_CUBIC = C12 / (C11 + C12)
 
 @dataclass
 class Material:
-    slip_set: int = 1           # 1 fcc, 8 bcc48, 80 bcc24 (bcc48's {110} and {112} systems)
+    slip_set: int = 1           # 1 fcc, 2 bcc, 6 roters, 7 bcc12, 8 bcc48 (the reference's
+                                # slip_type); 80 bcc24 (bcc48's {110} and {112} systems)
     e: float = E_CUBIC
     nu: float = NU_CUBIC
     mu: float = C44

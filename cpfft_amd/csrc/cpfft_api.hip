@@ -1058,7 +1058,7 @@ extern "C" int cpfft_counters(cpfft_ctx* c, long long out[8]) {
 extern "C" int cpfft_mlp_init(cpfft_ctx* c, uint64_t seed, long capacity, int batch) {
   if (!c) return fail(-1, "cpfft_mlp_init: null context");
   if (c->nslip > kMlpMaxOut)
-    return fail(-1, "cpfft_mlp_init: the warm start serves fcc (12) and bcc24 (24 slip systems)");
+    return fail(-1, "cpfft_mlp_init: the network serves crystals of 12 or 24 slip systems");
   if (capacity < 1 || batch < 16) return fail(-1, "cpfft_mlp_init: capacity >= 1 and batch >= 16 required");
   MlpBuffers& b = c->mlp;
   if (!c->mlp_ready) {

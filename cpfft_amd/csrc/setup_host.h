@@ -13,16 +13,27 @@
 namespace cpfft {
 
 // ------------------------------------------------------------------ setup
-// Crystal-frame slip systems as integer Miller triples: fcc
-// (mod_crystals.f:438-515) and bcc48 (mod_crystals.f:812-1207: {110}<111>,
-// {112}<111>, {123}<111>, in that order).  A unit vector has components
-// sign(v) |v|/sqrt(|v|^2), the reference's named factors (f3 = 1/sqrt(3),
-// f211 = 2/sqrt(6), f312 = 3/sqrt(14), ...) to the bit.
+// Crystal-frame slip systems as integer Miller triples, numbered as the
+// reference's slip_type: fcc (1, mod_crystals.f:438-515), bcc (2, :516-593:
+// {110}<111>, the fcc table with directions and normals exchanged), roters
+// (6, :604-704: fcc {111}<110> in another order and sign), bcc12 (7,
+// :705-811: the first 12 systems of bcc48) and bcc48 (8, :812-1207:
+// {110}<111>, {112}<111>, {123}<111>, in that order); 80 is the first 24
+// systems of bcc48.  A unit vector has components sign(v) |v|/sqrt(|v|^2),
+// the reference's named factors (f3 = 1/sqrt(3), f211 = 2/sqrt(6),
+// f312 = 3/sqrt(14), ...) to the bit.
 constexpr int kMaxSlip = 48;
-enum { SLIP_FCC = 1, SLIP_BCC48 = 8, SLIP_BCC24 = 80 };
+enum {
+  SLIP_FCC = 1, SLIP_BCC = 2, SLIP_ROTERS = 6, SLIP_BCC12 = 7, SLIP_BCC48 = 8, SLIP_BCC24 = 80
+};
 
 inline int slip_count(int set) {
-  return set == SLIP_FCC ? 12 : set == SLIP_BCC48 ? 48 : set == SLIP_BCC24 ? 24 : 0;
+  switch (set) {
+    case SLIP_FCC: case SLIP_BCC: case SLIP_ROTERS: case SLIP_BCC12: return 12;
+    case SLIP_BCC48: return 48;
+    case SLIP_BCC24: return 24;
+    default: return 0;
+  }
 }
 
 inline void unit_miller(const int v[3], double out[3]) {
@@ -33,8 +44,7 @@ inline void unit_miller(const int v[3], double out[3]) {
   }
 }
 
-// set: SLIP_FCC (12 systems), SLIP_BCC48 (48) or SLIP_BCC24 (the first 24
-// systems of bcc48, {110} and {112}); ns = 0 for another set
+// set: one of the SLIP_* sets above; ns = 0 for another set
 inline void slip_systems(int set, int& ns, double b[kMaxSlip][3], double n[kMaxSlip][3]) {
   static const int fb[12][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}, {0, 1, 1},  {1, 0, 1},  {1, -1, 0},
                                 {0, -1, 1}, {-1, 0, -1}, {1, 1, 0}, {0, 1, 1},  {1, 0, -1}, {-1, -1, 0}};
@@ -54,10 +64,20 @@ inline void slip_systems(int set, int& ns, double b[kMaxSlip][3], double n[kMaxS
       {1, 2, 3},  {-1, 2, 3}, {1, -2, 3}, {1, 2, -3}, {1, 3, 2},  {-1, 3, 2}, {1, -3, 2}, {1, 3, -2},
       {2, 1, 3},  {-2, 1, 3}, {2, -1, 3}, {2, 1, -3}, {2, 3, 1},  {-2, 3, 1}, {2, -3, 1}, {2, 3, -1},
       {3, 1, 2},  {-3, 1, 2}, {3, -1, 2}, {3, 1, -2}, {3, 2, 1},  {-3, 2, 1}, {3, -2, 1}, {3, 2, -1}};
+  static const int rb[12][3] = {{1, -1, 0}, {1, 0, -1}, {0, 1, -1}, {1, 1, 0}, {1, 0, 1},  {0, 1, -1},
+                                {1, 1, 0},  {1, 0, -1}, {0, 1, 1},  {1, -1, 0}, {1, 0, 1}, {0, 1, 1}};
+  static const int rn[12][3] = {{1, 1, 1},   {-1, -1, -1}, {1, 1, 1},   {1, -1, -1},
+                                {-1, 1, 1},  {1, -1, -1},  {1, -1, 1},  {1, -1, 1},
+                                {-1, 1, -1}, {-1, -1, 1},  {-1, -1, 1}, {1, 1, -1}};
   ns = slip_count(set);
+  const int(*tb)[3] = bb;
+  const int(*tn)[3] = bn;
+  if (set == SLIP_FCC) { tb = fb; tn = fn; }
+  else if (set == SLIP_BCC) { tb = fn; tn = fb; }
+  else if (set == SLIP_ROTERS) { tb = rb; tn = rn; }
   for (int s = 0; s < ns; ++s) {
-    unit_miller(set == SLIP_FCC ? fb[s] : bb[s], b[s]);
-    unit_miller(set == SLIP_FCC ? fn[s] : bn[s], n[s]);
+    unit_miller(tb[s], b[s]);
+    unit_miller(tn[s], n[s]);
   }
 }
 

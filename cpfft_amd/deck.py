@@ -43,8 +43,9 @@ import numpy as np
 
 from .configs import Material
 
-# slip_type names of incrystal.f:57-84 that this solver implements
-SLIP_TYPES = {"fcc": 1, "bcc48": 8}
+# slip_type names of incrystal.f:57-84 that this solver implements (all but
+# single, hcp6, hcp18 and halite)
+SLIP_TYPES = {"fcc": 1, "bcc": 2, "roters": 6, "bcc12": 7, "bcc48": 8}
 BC_NAMES = ["xx", "xy", "xz", "yx", "yy", "yz", "zx", "zy", "zz"]
 
 
@@ -133,7 +134,7 @@ class Deck:
         if cr.hardening != "voce":
             raise DeckError(f"hardening {cr.hardening} is not implemented (Voce only)")
         if cr.slip_type not in SLIP_TYPES.values():
-            raise DeckError(f"slip_type {cr.slip_type} is not implemented (fcc, bcc48)")
+            raise DeckError(f"slip_type {cr.slip_type} is not implemented ({', '.join(SLIP_TYPES)})")
         if cr.elastic_type == 1:
             mu = cr.e / (2.0 * (1.0 + cr.nu))
         elif cr.elastic_type == 2:
@@ -272,7 +273,7 @@ def _crystal_props(cr, toks):
         if k == "slip_type":
             name = toks[i + 1].lower()
             if name not in SLIP_TYPES:
-                raise DeckError(f"slip_type {name} is not implemented (fcc, bcc48)")
+                raise DeckError(f"slip_type {name} is not implemented ({', '.join(SLIP_TYPES)})")
             cr.slip_type = SLIP_TYPES[name]
             i += 2
         elif k == "elastic_type":

@@ -42,9 +42,10 @@ typedef struct cpfft_ctx cpfft_ctx;
  * Mirrors the crystal properties the deck command `crystal` sets
  * (src/incrystal.f, defaults src/mod_crystals.f:231-410). */
 typedef struct {
-  int32_t slip_set;   /* 1 = fcc {111}<110> (12); 8 = bcc48, {110}+{112}+{123}
-                         <111> (48), as the reference's slip_type 8; 80 = the
-                         first 24 systems of bcc48 ({110} and {112})        */
+  int32_t slip_set;   /* the reference's slip_type: 1 fcc {111}<110> (12),
+                         2 bcc {110}<111> (12), 6 roters (fcc, 12), 7 bcc12
+                         (12), 8 bcc48 {110}+{112}+{123}<111> (48); and 80,
+                         the first 24 systems of bcc48 ({110} and {112})    */
   double e, nu, mu;   /* cubic elasticity through the compliance (mod_crystals.f:1833-1879) */
   double rate_n;      /* harden_n: power-law rate exponent                   */
   double tau_y, tau_v, theta_0, voche_m;   /* Voce hardening                 */
@@ -288,7 +289,7 @@ int cpfft_mlp_train(cpfft_ctx* ctx, int iters, double lr, uint64_t seed, double*
  * (no counterpart in the reference; off by default, when results are those
  * of the reference's start).  mode is a bit set of the candidate starts:
  * CPFFT_WARM_NETWORK the network's predicted slip increments (needs
- * cpfft_mlp_init; fcc and bcc24), CPFFT_WARM_PREVIOUS the step's previous
+ * cpfft_mlp_init; 12 or 24 slip systems), CPFFT_WARM_PREVIOUS the step's previous
  * global iterate held in STATE_NP1.  With both, the network serves only
  * iiter == 1 (where the previous iterate is the explicit update of iiter
  * 0) and the previous iterate the later calls.  Per voxel the candidate of smaller

@@ -237,8 +237,10 @@ int orc_crystal_init(orc_crystal* c, int slip_set, double e, double nu, double m
                      double rate_n, double tau_y, double tau_v, double theta_0,
                      double voche_m, int alter_mode, double gamma_bar) {
   memset(c, 0, sizeof *c);
-  /* slip systems: mod_crystals.f:438-515 (slip_type 1, fcc) and 812-1207
-   * (slip_type 8, bcc48: {110}<111>, {112}<111>, {123}<111>); slip set 80
+  /* slip systems: mod_crystals.f:438-515 (slip_type 1, fcc), 516-593 (2,
+   * bcc: the fcc directions as normals and normals as directions), 604-704
+   * (6, roters), 705-811 (7, bcc12: the first 12 systems of bcc48) and
+   * 812-1207 (8, bcc48: {110}<111>, {112}<111>, {123}<111>); slip set 80
    * is the first 24 systems of bcc48.  The reference assigns each entry a
    * named factor (f = 1/sqrt(2), f3 = 1/sqrt(3), f211 = 2/sqrt(6),
    * f312 = 3/sqrt(14), ...): component v of a Miller triple of squared length
@@ -269,8 +271,18 @@ int orc_crystal_init(orc_crystal* c, int slip_set, double e, double nu, double m
       {1, -3, 2}, {1, 3, -2}, {2, 1, 3},  {-2, 1, 3}, {2, -1, 3}, {2, 1, -3},
       {2, 3, 1},  {-2, 3, 1}, {2, -3, 1}, {2, 3, -1}, {3, 1, 2},  {-3, 1, 2},
       {3, -1, 2}, {3, 1, -2}, {3, 2, 1},  {-3, 2, 1}, {3, -2, 1}, {3, 2, -1}};
+  /* roters (slip_type 6): directions <110>, normals {111} */
+  static const int rb[12][3] = {{1, -1, 0}, {1, 0, -1}, {0, 1, -1}, {1, 1, 0},
+                                {1, 0, 1},  {0, 1, -1}, {1, 1, 0},  {1, 0, -1},
+                                {0, 1, 1},  {1, -1, 0}, {1, 0, 1},  {0, 1, 1}};
+  static const int rn[12][3] = {{1, 1, 1},   {-1, -1, -1}, {1, 1, 1},   {1, -1, -1},
+                                {-1, 1, 1},  {1, -1, -1},  {1, -1, 1},  {1, -1, 1},
+                                {-1, 1, -1}, {-1, -1, 1},  {-1, -1, 1}, {1, 1, -1}};
   const int(*tb)[3], (*tn)[3];
   if (slip_set == 1) { c->nslip = 12; tb = fb; tn = fn; }
+  else if (slip_set == 2) { c->nslip = 12; tb = fn; tn = fb; }
+  else if (slip_set == 6) { c->nslip = 12; tb = rb; tn = rn; }
+  else if (slip_set == 7) { c->nslip = 12; tb = bb; tn = bn; }
   else if (slip_set == 8) { c->nslip = 48; tb = bb; tn = bn; }
   else if (slip_set == 80) { c->nslip = 24; tb = bb; tn = bn; }
   else return -1;

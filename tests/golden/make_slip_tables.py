@@ -1,12 +1,12 @@
 """Extract the crystal-frame slip systems (directions b and plane normals n,
-as integer Miller-index triples) of the reference's slip types 1 (fcc) and
-8 (bcc48) from its constant assignments (src/mod_crystals.f:438-515 and
-:812-1207) into slip_tables.json, the fixture the slip-table tests compare
-the oracle's and the library's tables with.
+as integer Miller-index triples) of the reference's slip types 1 (fcc),
+2 (bcc), 6 (roters), 7 (bcc12) and 8 (bcc48) from its constant assignments
+(src/mod_crystals.f:438-1207) into slip_tables.json, the fixture the
+slip-table tests compare the oracle's and the library's tables with.
 
 Each assignment reads `c_array(num)%bi(s,k)=<value>` with <value> one of
 the named scale factors of the block (f, f2, f3, f112, f211, f123, f213,
-f312, z0 or 0, possibly negated); a factor is turned back into its integer
+f312, z0 or 0, possibly negated) or a literal such as 0.d0; a factor is turned back into its integer
 multiple of the block's unit 1/sqrt(m) (m = |index|^2), so the fixture holds
 small integers only.  Run in the container that holds the reference:
     python tests/golden/make_slip_tables.py
@@ -41,7 +41,8 @@ def block(lines, first, last):
         m = ASSIGN.search(ln)
         if m:
             arr, s, k, neg, name = m.groups()
-            v = fac[name] * (-1.0 if neg else 1.0)
+            base = fac[name] if name in fac else fnum(name)  # a named factor or a literal
+            v = base * (-1.0 if neg else 1.0)
             tab[arr].setdefault(int(s), [0.0, 0.0, 0.0])[int(k) - 1] = v
     ns = max(tab["bi"])
     out = {}
@@ -64,16 +65,19 @@ def block(lines, first, last):
 
 def main():
     lines = open(SRC).read().splitlines()
-    fcc = block(lines, 438, 515)
-    bcc48 = block(lines, 812, 1207)
-    assert len(fcc["bi"]) == 12 and len(bcc48["bi"]) == 48
-    data = {"source": "src/mod_crystals.f:438-515 (slip_type 1, fcc), :812-1207 (slip_type 8, bcc48)",
-            "fcc": {"b": fcc["bi"], "n": fcc["ni"]},
-            "bcc48": {"b": bcc48["bi"], "n": bcc48["ni"]}}
+    blocks = {"fcc": (438, 515, 12), "bcc": (516, 593, 12), "roters": (604, 704, 12),
+              "bcc12": (705, 811, 12), "bcc48": (812, 1207, 48)}
+    data = {"source": "src/mod_crystals.f:438-515 (slip_type 1, fcc), :516-593 (2, bcc), "
+                      ":604-704 (6, roters), :705-811 (7, bcc12), :812-1207 (8, bcc48)"}
+    for name, (a, b, ns) in blocks.items():
+        t = block(lines, a, b)
+        assert len(t["bi"]) == ns, name
+        data[name] = {"b": t["bi"], "n": t["ni"]}
     with open(os.path.join(HERE, "slip_tables.json"), "w") as f:
         json.dump(data, f, indent=None)
         f.write("\n")
-    print("fcc 12, bcc48 48 systems written")
+    print(", ".join(f"{k} {len(v['b'])}" for k, v in data.items() if k != "source"),
+          "systems written")
 
 
 if __name__ == "__main__":

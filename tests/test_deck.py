@@ -129,3 +129,22 @@ def test_reference_case_fixture_is_the_oracle_run():
     # every component of F prescribed: mean F is the prescription and P
     # rises with the tension
     assert (np.diff(P[:, 0]) > 0).all()
+
+
+@pytest.mark.parametrize("name,slip_set", [("fcc", 1), ("bcc", 2), ("roters", 6), ("bcc12", 7),
+                                           ("bcc48", 8)])
+def test_slip_type_names(tmp_path, name, slip_set):
+    """incrystal.f:57-84's slip_type names map to the library's slip sets;
+    the ones it does not implement are refused."""
+    src = open(os.path.join(CASE, "test_mm10.in")).read().replace("slip_type bcc48", f"slip_type {name}")
+    p = tmp_path / "d.in"
+    p.write_text(src)
+    assert read_deck(str(p)).material().slip_set == slip_set
+
+
+def test_unimplemented_slip_type_is_refused(tmp_path):
+    src = open(os.path.join(CASE, "test_mm10.in")).read().replace("slip_type bcc48", "slip_type hcp6")
+    p = tmp_path / "d.in"
+    p.write_text(src)
+    with pytest.raises(DeckError):
+        read_deck(str(p)).material()

@@ -88,16 +88,18 @@ def _fields_near_identity(rng, V, amp, uniaxial=0.0):
 
 # seed offset per slip set (the BCC24 cases keep the seeds they had when
 # that set was numbered 8)
-SEED_OF = {1: 1, 80: 8, 8: 48}
+SEED_OF = {1: 1, 80: 8, 8: 48, 2: 2, 6: 6, 7: 7}
 
 
-@pytest.mark.parametrize("slip_set,jac_exact", [(1, 0), (1, 1), (80, 1), (8, 0)])
+@pytest.mark.parametrize("slip_set,jac_exact",
+                         [(1, 0), (1, 1), (80, 1), (8, 0), (2, 1), (6, 0), (7, 1)])
 def test_drive_eps_sig(slip_set, jac_exact):
     """Two load steps of per-voxel updates on 8^3 voxels: elastic predictor,
     plastic Newton updates, a second step from the first step's state.
     slip_set 1: FCC (12 systems); 80: BCC24 (24 systems, the 256^3
     workload's crystal); 8: bcc48 (48 systems, the reference's own example
-    deck's crystal) with the reference's J22."""
+    deck's crystal) with the reference's J22; 2, 6, 7: the reference's bcc,
+    roters and bcc12 tables (12 systems each)."""
     N = 8
     V = N ** 3
     rng = np.random.default_rng(2 + jac_exact + SEED_OF[slip_set])

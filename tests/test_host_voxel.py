@@ -77,10 +77,11 @@ def check(a, b, nslip, what):
 
 # seed offset per slip set: 1 fcc, 80 bcc24 (numbered 8 when its seeds were
 # chosen), 8 bcc48
-SEED_OF = {1: 1, 80: 8, 8: 48}
+SEED_OF = {1: 1, 80: 8, 8: 48, 2: 2, 6: 6, 7: 7}
 
 
-@pytest.mark.parametrize("slip_set,jac_exact", [(1, 0), (1, 1), (80, 1), (8, 0)])
+@pytest.mark.parametrize("slip_set,jac_exact",
+                         [(1, 0), (1, 1), (80, 1), (8, 0), (2, 1), (6, 0), (7, 1)])
 def test_host_update_matches_oracle(slip_set, jac_exact):
     V = 6 ** 3
     rng = np.random.default_rng(40 + SEED_OF[slip_set] + jac_exact)

@@ -4,10 +4,11 @@ oracle cannot pin when both read the same table the same way).
 
 * Slip tables: the oracle's and the library's crystal-frame tables equal
   tests/golden/slip_tables.json, which make_slip_tables.py extracts
-  mechanically from the reference's assignments (mod_crystals.f:438-515,
-  812-1207); and independently of the reference text they are complete
-  crystallographic families: unit vectors, b . n = 0, fcc {111}<110> (12),
-  bcc {110}<111> (12), {112}<111> (12), {123}<111> (24), no system twice.
+  mechanically from the reference's assignments (mod_crystals.f:438-1207,
+  slip types fcc, bcc, roters, bcc12 and bcc48); and independently of the
+  reference text they are complete crystallographic families: unit
+  vectors, b . n = 0, fcc {111}<110> (12), bcc {110}<111> (12),
+  {112}<111> (12), {123}<111> (24), no system twice.
 * Stiffness: inverting the compliance of the cubic inputs (e, nu, mu)
   returns C11/C12/C44 = 168.4/121.4/75.4 GPa; isotropic inputs return the
   Lame form; the library's rotated stiffness, Schmid tensors and spin
@@ -47,7 +48,8 @@ def unit_rows(a):
 
 
 def expected_table(slip_set):
-    key, ns = {1: ("fcc", 12), 8: ("bcc48", 48), 80: ("bcc48", 24)}[slip_set]
+    key, ns = {1: ("fcc", 12), 2: ("bcc", 12), 6: ("roters", 12), 7: ("bcc12", 12),
+               8: ("bcc48", 48), 80: ("bcc48", 24)}[slip_set]
     return unit_rows(TABLES[key]["b"][:ns]), unit_rows(TABLES[key]["n"][:ns])
 
 
@@ -73,7 +75,7 @@ def library_table(slip_set):
 
 
 # ---------------------------------------------------------------- slip tables
-@pytest.mark.parametrize("slip_set", [1, 8, 80])
+@pytest.mark.parametrize("slip_set", [1, 2, 6, 7, 8, 80])
 def test_slip_tables_match_the_reference_tables(slip_set):
     B, N = expected_table(slip_set)
     ob, on = oracle_table(O.crystal(slip_set=slip_set))
@@ -90,6 +92,9 @@ def _family(v):
 
 @pytest.mark.parametrize("key,families", [
     ("fcc", [((1, 1, 1), (0, 1, 1), 12)]),
+    ("bcc", [((0, 1, 1), (1, 1, 1), 12)]),
+    ("roters", [((1, 1, 1), (0, 1, 1), 12)]),
+    ("bcc12", [((0, 1, 1), (1, 1, 1), 12)]),
     ("bcc48", [((0, 1, 1), (1, 1, 1), 12), ((1, 1, 2), (1, 1, 1), 12),
                ((1, 2, 3), (1, 1, 1), 24)])])
 def test_slip_tables_are_complete_families(key, families):
