@@ -844,11 +844,14 @@ __global__ __launch_bounds__(64) void k_mlp_features_norm(long V, MatParams P,
                      [&](int which, int c) { return which ? Fn1[c * V + e] : Fn[c * V + e]; },
                      [&](int w) { return sn[w * V + e]; }, f);
   uint16_t* x = Xv + e * ninp;
-  for (int k = 0; k < ninp; k += 2) {
-    const uint16_t lo = k < NIN ? f32_to_bf16((f[k] - nrm[k]) / nrm[NIN + k]) : 0;
-    const uint16_t hi = k + 1 < NIN ? f32_to_bf16((f[k + 1] - nrm[k + 1]) / nrm[NIN + k + 1]) : 0;
+  // NIN is even; compile-time indexes keep f in registers
+#pragma unroll
+  for (int k = 0; k < NIN; k += 2) {
+    const uint16_t lo = f32_to_bf16((f[k] - nrm[k]) / nrm[NIN + k]);
+    const uint16_t hi = f32_to_bf16((f[k + 1] - nrm[k + 1]) / nrm[NIN + k + 1]);
     *reinterpret_cast<uint32_t*>(x + k) = (uint32_t)lo | ((uint32_t)hi << 16);
   }
+  for (int k = NIN; k < ninp; k += 2) *reinterpret_cast<uint32_t*>(x + k) = 0u;
 }
 
 template <int NS>

@@ -1024,18 +1024,117 @@ CPFFT_HD inline void warm_x0(const MatParams& P, const Orient<NS>& o, const Poin
   x0[6] = tt0;
 }
 
+// ---- the network's inputs, in FP32 (the network reads them in BF16)
+
+// inverse of a 3x3 FP32 matrix by cofactors
+CPFFT_HD inline void inv33f(const float a[3][3], float g[3][3]) {
+  g[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+  g[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+  g[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+  g[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+  g[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+  g[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+  g[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  g[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+  g[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+  const float id = 1.0f / (a[0][0] * g[0][0] + a[0][1] * g[1][0] + a[0][2] * g[2][0]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[i][j] *= id;
+}
+
+// The unrotated strain increment uddt of step_kinematics to FP32 accuracy:
+// F_{n+1/2} and dF formed in FP64, the rotation of F_{n+1/2} by Higham's
+// Newton iteration R <- (R + R^-T)/2 from R = F (quadratic convergence:
+// five steps reach FP32 accuracy for stretches up to about 40%), then the
+// same D = sym(dF F_{n+1/2}^-1) and Voigt rotation as step_kinematics.
+CPFFT_HD inline void uddt_f32(const double Fn[3][3], const double Fn1[3][3], float uddt[6]) {
+  float fnh[3][3], dfn[3][3], r[3][3], ri[3][3], B[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      fnh[i][j] = (float)(0.5 * (Fn[i][j] + Fn1[i][j]));
+      dfn[i][j] = (float)(Fn1[i][j] - Fn[i][j]);
+      r[i][j] = fnh[i][j];
+    }
+#pragma unroll
+  for (int it = 0; it < 5; ++it) {
+    inv33f(r, ri);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r[i][j] = 0.5f * (r[i][j] + ri[j][i]);
+  }
+  inv33f(fnh, B);
+  const float(*A)[3] = dfn;
+  float ddt[6];
+  ddt[0] = A[0][0] * B[0][0] + A[0][1] * B[1][0] + A[0][2] * B[2][0];
+  ddt[3] = A[1][0] * B[0][0] + A[1][1] * B[1][0] + A[1][2] * B[2][0] + A[0][0] * B[0][1] +
+           A[0][1] * B[1][1] + A[0][2] * B[2][1];
+  ddt[5] = A[2][0] * B[0][0] + A[2][1] * B[1][0] + A[2][2] * B[2][0] + A[0][0] * B[0][2] +
+           A[0][1] * B[1][2] + A[0][2] * B[2][2];
+  ddt[1] = A[1][0] * B[0][1] + A[1][1] * B[1][1] + A[1][2] * B[2][1];
+  ddt[4] = A[2][0] * B[0][1] + A[2][1] * B[1][1] + A[2][2] * B[2][1] + A[1][0] * B[0][2] +
+           A[1][1] * B[1][2] + A[1][2] * B[2][2];
+  ddt[2] = A[2][0] * B[0][2] + A[2][1] * B[1][2] + A[2][2] * B[2][2];
+  float q[6][6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    q[a][0] = r[0][a] * r[0][a];
+    q[a][1] = r[1][a] * r[1][a];
+    q[a][2] = r[2][a] * r[2][a];
+    q[a][3] = r[0][a] * r[1][a];
+    q[a][4] = r[2][a] * r[1][a];
+    q[a][5] = r[0][a] * r[2][a];
+  }
+  const int ia[3] = {0, 1, 0}, ib[3] = {1, 2, 2};
+#pragma unroll
+  for (int z = 0; z < 3; ++z) {
+    const int a = ia[z], b = ib[z];
+    q[3 + z][0] = 2.0f * r[0][a] * r[0][b];
+    q[3 + z][1] = 2.0f * r[1][a] * r[1][b];
+    q[3 + z][2] = 2.0f * r[2][a] * r[2][b];
+    q[3 + z][3] = r[0][a] * r[1][b] + r[0][b] * r[1][a];
+    q[3 + z][4] = r[1][a] * r[2][b] + r[2][a] * r[1][b];
+    q[3 + z][5] = r[0][a] * r[2][b] + r[2][a] * r[0][b];
+  }
+  // the reference's index pattern for rows 4..6 (polar.f:735-758)
+  q[4][1] = 2.0f * r[1][2] * r[1][1];
+  q[4][2] = 2.0f * r[2][1] * r[2][2];
+  q[4][3] = r[0][1] * r[1][2] + r[1][1] * r[0][2];
+  q[4][5] = r[0][1] * r[2][2] + r[2][1] * r[0][2];
+  q[5][3] = r[0][0] * r[1][2] + r[1][0] * r[0][2];
+  q[3][4] = r[1][0] * r[2][1] + r[2][0] * r[1][1];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    uddt[j] = q[j][0] * ddt[0] + q[j][1] * ddt[1] + q[j][2] * ddt[2] + q[j][3] * ddt[3] +
+              q[j][4] * ddt[4] + q[j][5] * ddt[5];
+}
+
+// slip_to_u in FP32 after an exact binary split |g|/dg = m 2^e (FP64 slip
+// increments reach 1e-50 and below, out of FP32's range): 2^((e + log2 m)/n)
+CPFFT_HD inline float slip_to_u_f32(const MatParams& P, double dg, double g) {
+  if (g == 0.0) return 0.0f;
+  int e;
+  const double m = frexp(fabs(g) / dg, &e);  // m in [0.5, 1)
+  const float u = exp2f(((float)e + log2f((float)m)) / (float)P.rate_n);
+  return g < 0.0 ? -u : u;
+}
+
 // Inputs of the warm-start network for one voxel (18 + 2 NS values: 42 for
-// fcc, 66 for bcc24): the trial stress s_n + C:D rotated to the lattice
-// frame (Rp_n^T S Rp_n as a full 3x3, row-major), the slip resistance tt_n
-// once per slip system (the Voce law keeps a single one), the strain
-// increment D in the lattice frame (3x3 tensor, row-major), and the slip
-// increments of the previous step (as u_s, see slip_to_u).
+// fcc, 66 for bcc24), in FP32: the trial stress s_n + C:D rotated to the
+// lattice frame (Rp_n^T S Rp_n as a full 3x3, row-major), the slip
+// resistance tt_n once per slip system (the Voce law keeps a single one),
+// the strain increment D in the lattice frame (3x3 tensor, row-major), and
+// the slip increments of the previous step (as u_s, see slip_to_u).
 template <int NS, class LoadF, class LoadS>
 CPFFT_HD inline void voxel_features(const MatParams& P, const Orient<NS>& o, int step, LoadF&& ldF,
                                     LoadS&& ldn, float* feat) {
-  double uddt[6], rn1[3][3];
+  float uddt[6];
   {
-    double Fn[3][3], Fn1[3][3], rnh[3][3], fnhinv[3][3];
+    double Fn[3][3], Fn1[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -1043,55 +1142,59 @@ CPFFT_HD inline void voxel_features(const MatParams& P, const Orient<NS>& o, int
         Fn[i][j] = ldF(0, 3 * i + j);
         Fn1[i][j] = ldF(1, 3 * i + j);
       }
-    step_kinematics(Fn, Fn1, rnh, rn1, fnhinv, uddt);
+    uddt_f32(Fn, Fn1, uddt);
   }
-  double sn[6], Rp[3][3], tt;
+  float sn[6], Rp[3][3], tt;
   if (step == 1) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) sn[k] = 0.0;
+    for (int k = 0; k < 6; ++k) sn[k] = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) Rp[i][j] = (i == j) ? 1.0 : 0.0;
-    tt = P.tau_y + 1.0e-5;
+      for (int j = 0; j < 3; ++j) Rp[i][j] = (i == j) ? 1.0f : 0.0f;
+    tt = (float)(P.tau_y + 1.0e-5);
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) sn[k] = ldn(k);
+    for (int k = 0; k < 6; ++k) sn[k] = (float)ldn(k);
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) Rp[i][j] = ldn(6 + 3 * j + i);
-    tt = ldn(15);
+      for (int i = 0; i < 3; ++i) Rp[i][j] = (float)ldn(6 + 3 * j + i);
+    tt = (float)ldn(15);
   }
-  double c[6];
-  Cv6(o.C, uddt, c);
-  const double st[6] = {sn[0] + c[0], sn[1] + c[1], sn[2] + c[2],
-                        sn[3] + c[3], sn[4] + c[4], sn[5] + c[5]};
-  const double S[3][3] = {{st[0], st[3], st[5]}, {st[3], st[1], st[4]}, {st[5], st[4], st[2]}};
-  const double D[3][3] = {{uddt[0], 0.5 * uddt[3], 0.5 * uddt[5]},
-                          {0.5 * uddt[3], uddt[1], 0.5 * uddt[4]},
-                          {0.5 * uddt[5], 0.5 * uddt[4], uddt[2]}};
+  float st[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    float c = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) c += (float)o.C[a * 6 + b] * uddt[b];
+    st[a] = sn[a] + c;
+  }
+  const float S[3][3] = {{st[0], st[3], st[5]}, {st[3], st[1], st[4]}, {st[5], st[4], st[2]}};
+  const float D[3][3] = {{uddt[0], 0.5f * uddt[3], 0.5f * uddt[5]},
+                         {0.5f * uddt[3], uddt[1], 0.5f * uddt[4]},
+                         {0.5f * uddt[5], 0.5f * uddt[4], uddt[2]}};
   // lattice frame: Rp^T X Rp (tn = Rp^T maps lattice to unrotated)
+  float SR[3][3], DR[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      SR[k][j] = S[k][0] * Rp[0][j] + S[k][1] * Rp[1][j] + S[k][2] * Rp[2][j];
+      DR[k][j] = D[k][0] * Rp[0][j] + D[k][1] * Rp[1][j] + D[k][2] * Rp[2][j];
+    }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-          a += Rp[k][i] * S[k][l] * Rp[l][j];
-          b += Rp[k][i] * D[k][l] * Rp[l][j];
-        }
-      feat[3 * i + j] = (float)a;
-      feat[9 + NS + 3 * i + j] = (float)b;
+      feat[3 * i + j] = Rp[0][i] * SR[0][j] + Rp[1][i] * SR[1][j] + Rp[2][i] * SR[2][j];
+      feat[9 + NS + 3 * i + j] = Rp[0][i] * DR[0][j] + Rp[1][i] * DR[1][j] + Rp[2][i] * DR[2][j];
     }
 #pragma unroll
-  for (int s = 0; s < NS; ++s) feat[9 + s] = (float)tt;
+  for (int s = 0; s < NS; ++s) feat[9 + s] = tt;
   const double dg = P.alter_mode ? P.eps_dot_0_y * P.dt : 1.0;
   for (int s = 0; s < NS; ++s)
-    feat[18 + NS + s] = step == 1 ? 0.0f : (float)slip_to_u(P, dg, ldn(23 + s));
+    feat[18 + NS + s] = step == 1 ? 0.0f : slip_to_u_f32(P, dg, ldn(23 + s));
 }
 
 // Per-voxel stress update.  ldF(0|1, 3i+j) loads F_n | F_{n+1}; ldn(w)

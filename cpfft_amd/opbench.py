@@ -139,6 +139,18 @@ def main():
         avg, n = s.kernel_times()
         res["drive_eps_sig_ms"] = avg[1]
         res["voxel_updates_per_s"] = V / (avg[1] * 1e-3)
+        if args.mlp:
+            # the warm start's inference path (inputs, then the fused network)
+            # on every voxel, from the rolled-over plastic state; the wall time
+            # includes the host copies of inputs and outputs
+            s.update()
+            s.upload(FN1, np.tile(np.array([1.001, 0, 0, 0, 0.9997, 0, 0, 0, 0.9997])[:, None],
+                                  (1, V)))
+            s.mlp_predict(2)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                s.mlp_predict(2)
+            res["mlp_predict_ms_wall"] = (time.perf_counter() - t0) * 1e3 / 3
     s.close()
     print(json.dumps(res), flush=True)
 

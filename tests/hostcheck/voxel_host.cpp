@@ -135,3 +135,83 @@ extern "C" int cpfft_host_orientation(const cpfft_material* m, const double* ang
   }
   return ns;
 }
+
+// The network inputs of every voxel (voxel_features, FP32) beside the same
+// quantities in FP64: step_kinematics's uddt, the trial stress s_n + C:D and
+// D rotated to the lattice frame Rp^T X Rp, tt_n, and slip_to_u of the
+// previous slip increments (words 23..).  f32 / f64 are [V][18 + 2 NS].
+template <int NS>
+static void features(const cpfft_material& m, double dt, long V, const double* ang, int step,
+                     const double* Fn, const double* Fn1, const double* sn, float* f32,
+                     double* f64) {
+  double b[kMaxSlip][3], nn[kMaxSlip][3], st[6][6];
+  int ns = 0;
+  slip_systems(m.slip_set, ns, b, nn);
+  cubic_stiffness(m, st);
+  const MatParams P = make_params(m, dt);
+  constexpr int NIN = 18 + 2 * NS;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long e = 0; e < V; ++e) {
+    Orient<NS> o;
+    const double a[3] = {ang[e], ang[V + e], ang[2 * V + e]};
+    build_orient<NS>(st, b, nn, a, o);
+    auto ldn = [&](int w) { return sn[w * V + e]; };
+    voxel_features<NS>(
+        P, o, step, [&](int which, int k) { return which ? Fn1[k * V + e] : Fn[k * V + e]; }, ldn,
+        f32 + e * NIN);
+    double F0[3][3], F1[3][3], rnh[3][3], rn1[3][3], fi[3][3], ud[6];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        F0[i][j] = Fn[(3 * i + j) * V + e];
+        F1[i][j] = Fn1[(3 * i + j) * V + e];
+      }
+    step_kinematics(F0, F1, rnh, rn1, fi, ud);
+    double s0[6] = {0, 0, 0, 0, 0, 0}, Rp[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double tt = P.tau_y + 1.0e-5;
+    if (step > 1) {
+      for (int k = 0; k < 6; ++k) s0[k] = ldn(k);
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) Rp[i][j] = ldn(6 + 3 * j + i);
+      tt = ldn(15);
+    }
+    double strial[6];
+    for (int i = 0; i < 6; ++i) {
+      strial[i] = s0[i];
+      for (int k = 0; k < 6; ++k) strial[i] += o.C[6 * i + k] * ud[k];
+    }
+    const double S[3][3] = {{strial[0], strial[3], strial[5]},
+                            {strial[3], strial[1], strial[4]},
+                            {strial[5], strial[4], strial[2]}};
+    const double D[3][3] = {{ud[0], 0.5 * ud[3], 0.5 * ud[5]},
+                            {0.5 * ud[3], ud[1], 0.5 * ud[4]},
+                            {0.5 * ud[5], 0.5 * ud[4], ud[2]}};
+    double* g = f64 + e * NIN;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double vs = 0.0, vd = 0.0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            vs += Rp[k][i] * S[k][l] * Rp[l][j];
+            vd += Rp[k][i] * D[k][l] * Rp[l][j];
+          }
+        g[3 * i + j] = vs;
+        g[9 + NS + 3 * i + j] = vd;
+      }
+    const double dg = P.alter_mode ? P.eps_dot_0_y * P.dt : 1.0;
+    for (int s = 0; s < NS; ++s) {
+      g[9 + s] = tt;
+      g[18 + NS + s] = step == 1 ? 0.0 : slip_to_u(P, dg, ldn(23 + s));
+    }
+  }
+}
+
+extern "C" int cpfft_host_features(const cpfft_material* m, double dt, long V,
+                                   const double* angles, int step, const double* Fn,
+                                   const double* Fn1, const double* sn, float* f32, double* f64) {
+  switch (slip_count(m->slip_set)) {
+    case 12: features<12>(*m, dt, V, angles, step, Fn, Fn1, sn, f32, f64); break;
+    case 24: features<24>(*m, dt, V, angles, step, Fn, Fn1, sn, f32, f64); break;
+    default: return -1;
+  }
+  return 0;
+}

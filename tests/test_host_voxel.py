@@ -190,3 +190,83 @@ def test_warm_start_from_previous_iterate_cuts_local_iterations(slip_set):
     # the consistent tangent is formed at the second-to-last local iterate
     # (mm10_a.f:1134-1142), which differs between the two solves
     assert rel(K4, Kp) < 1e-3
+
+
+def rotations(rng, V):
+    """V random rotation matrices as [9][V] (row-major per voxel)"""
+    q = rng.standard_normal((V, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)])
+    return R
+
+
+def matmul9(A, B):
+    return np.einsum("ikv,kjv->ijv", A.reshape(3, 3, -1), B.reshape(3, 3, -1)).reshape(9, -1)
+
+
+@pytest.mark.parametrize("slip_set", [1, 80])
+def test_network_inputs_match_fp64(slip_set):
+    """The warm-start network's inputs (mm10_core.h voxel_features, FP32:
+    rotation of F_n+1/2 by Higham's iteration, FP32 lattice rotations and
+    slip_to_u) against the same quantities in FP64 (step_kinematics's polar
+    rotation, FP64 slip_to_u), at a first step, at a plastic second step,
+    and under finite rotations and 30% stretches.  The network reads them in
+    BF16 (relative rounding 2^-9), so 1e-5 of each block's scale is far
+    below what it can see."""
+    L = host_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.cpfft_host_features.argtypes = [ctypes.POINTER(C.CMaterial), ctypes.c_double, ctypes.c_long,
+                                      dp, ctypes.c_int, dp, dp, dp,
+                                      ctypes.POINTER(ctypes.c_float), dp]
+    V = 6 ** 3
+    rng = np.random.default_rng(70 + slip_set)
+    _, ang = polycrystal(6, 5, 21 + slip_set)
+    mat = C.Material(slip_set=slip_set, jac_exact=1)
+    oc = O.crystal(slip_set=slip_set, jac_exact=1)
+    ns = oc.nslip
+    nin = 18 + 2 * ns
+    cm = C.CMaterial(**{k: getattr(mat, k) for k, _ in C.CMaterial._fields_})
+    dt = 0.25
+
+    def feats(step, Fn, Fn1, sn):
+        f32 = np.zeros((V, nin), np.float32)
+        f64 = np.zeros((V, nin))
+        rc = L.cpfft_host_features(ctypes.byref(cm), dt, V, P(np.ascontiguousarray(ang)), step,
+                                   P(np.ascontiguousarray(Fn)), P(np.ascontiguousarray(Fn1)),
+                                   P(np.ascontiguousarray(sn)),
+                                   f32.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), P(f64))
+        assert rc == 0
+        return f32.astype(np.float64), f64
+
+    def blocks_agree(f32, f64, what):
+        for lo, hi, name in [(0, 9, "stress"), (9, 9 + ns, "tt"), (9 + ns, 18 + ns, "D"),
+                             (18 + ns, nin, "u")]:
+            scale = abs(f64[:, lo:hi]).max()
+            if scale == 0.0:
+                assert abs(f32[:, lo:hi]).max() == 0.0, (what, name)
+                continue
+            err = abs(f32[:, lo:hi] - f64[:, lo:hi]).max() / scale
+            assert err < 1e-5, (what, name, err)
+
+    Fn = near_identity(rng, V, 0.0)
+    Fn1 = near_identity(rng, V, 1e-4, uniaxial=6e-4)
+    sn = O.state_init(oc, V)
+    blocks_agree(*feats(1, Fn, Fn1, sn), "step 1")
+    _, _, s1, cnt = host_drive(mat, dt, ang, 1, 1, Fn, Fn1, sn)
+    Fn, Fn1 = Fn1, Fn1 + near_identity(rng, V, 2e-5, uniaxial=2.5e-4) - near_identity(rng, V, 0.0)
+    _, _, s2, cnt = host_drive(mat, dt, ang, 2, 1, Fn, Fn1, s1)
+    assert cnt[1] > 0  # the state carries slip
+    f32, f64 = feats(3, Fn1, Fn1 + near_identity(rng, V, 2e-5, uniaxial=2.5e-4)
+                     - near_identity(rng, V, 0.0), s2)
+    assert abs(f64[:, 18 + ns:]).max() > 0.1  # non-trivial u of the slipping systems
+    blocks_agree(f32, f64, "plastic step")
+    # finite rotation and stretch: F = R (I + 0.3 sym noise + stretch)
+    R = rotations(rng, V)
+    U = near_identity(rng, V, 0.05, uniaxial=0.3)
+    U = 0.5 * (U + U.reshape(3, 3, V).transpose(1, 0, 2).reshape(9, V))
+    Fa = matmul9(R, U)
+    Fb = matmul9(R, U + near_identity(rng, V, 1e-4, uniaxial=5e-4) - near_identity(rng, V, 0.0))
+    blocks_agree(*feats(3, Fa, Fb, s2), "rotated and stretched")
