@@ -33,11 +33,12 @@ def solver(N, mode, ngr=4, seed=7, mat=None, dt=0.5):
     return C.Solver(N, ang, mat or C.Material(), dt=dt, gamma_mode=mode), ang
 
 
-@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32, 48, 64, 96])
+@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 12, 16, 32, 48, 64, 96, 128])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_gkdf(N, mode):
     """Generic runtime-plan path (N = 4..12) and compile-time-planned path
-    (16, 32, 48, 64) against the oracle's direct DFT."""
+    (16 ... 128; pass C in registers at 64 and 128) against the oracle's
+    direct DFT."""
     rng = np.random.default_rng(10 * N + mode)
     s, _ = solver(N, mode)
     fo = O.FFT(N, mode)
@@ -50,6 +51,29 @@ def test_gkdf(N, mode):
     s.gkdf(C.DFM, C.B, 1)
     assert rel(s.download(C.B), fo.gkdf(F, K4)) < 1e-12
     s.close()
+
+
+def test_gkdf_256():
+    """The planned path at 256^3 (config 5's grid: radix 8-8-4, pass C in
+    registers) against the oracle's direct DFT, with and without K4, for
+    both operators on the same inputs."""
+    N = 256
+    rng = np.random.default_rng(2560)
+    F = rng.standard_normal((9, N ** 3))
+    K4 = rng.standard_normal((81, N ** 3))
+    for mode in (1, 0):
+        s, _ = solver(N, mode)
+        fo = O.FFT(N, mode)
+        s.upload(C.DFM, F)
+        s.upload(C.K4, K4)
+        s.gkdf(C.DFM, C.B, 0)
+        e0 = rel(s.download(C.B), fo.gkdf(F))
+        s.gkdf(C.DFM, C.B, 1)
+        e1 = rel(s.download(C.B), fo.gkdf(F, K4))
+        print(f"\n[gkdf] N=256 mode {mode}: rel {e0:.2e}, with K4 {e1:.2e}", flush=True)
+        assert e0 < 1e-12 and e1 < 1e-12
+        s.close()
+        del fo
 
 
 def test_blas_and_reductions():
