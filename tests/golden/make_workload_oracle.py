@@ -12,13 +12,23 @@ tests check full-size runs without minutes of CPU work at test time:
                      count, loading direction and strain rate, every
                      component of F prescribed): per member its index in
                      the batch, P per step and the counts per step
+  c3_128_oracle.npz  the first two load steps of the 128^3, 256-grain FCC
+                     workload (configs c3_128, the benchmark's): the inputs,
+                     P_bar and the counts per step, and the final P and
+                     material state of every 2048th voxel
+  c5_cyclic_32_oracle.npz  config 5's material and loading (BCC24, cyclic
+                     +-1.5% in steps of 2.5e-4, so the first reversal after
+                     step 60) on a 32^3, 16-grain RVE: the first 70 load
+                     steps, P_bar and the counts per step, with the
+                     workload's parameters
   c4_n48_n96_oracle.npz  the first load step of the first 48^3 and the
                      first 96^3 member (the radix-3 transforms), with P
                      and the material state of every 256th voxel
 
 The oracle is oracle/ (the CPU restatement of FFT_nr3 and the mm10
 update).  Run from the repository root after building the oracle:
-    python tests/golden/make_workload_oracle.py [c2|c4|c4big]
+    python tests/golden/make_workload_oracle.py [c2|c3|c4|c4big|c5small]
+(c3 takes about half an hour of 8 host cores per load step).
 """
 import os
 import sys
@@ -54,6 +64,36 @@ def c2():
     print("c2_64 P_bar_xx:", P[:, 0].tolist(), "counters:", cnt.tolist())
 
 
+def c3():
+    wl = WORKLOADS["c3_128"]
+    BC, isNBC, ang, P, cnt, st = run(wl)
+    idx = np.arange(0, wl.N ** 3, 2048)
+    np.savez_compressed(os.path.join(HERE, "c3_128_oracle.npz"), BC=BC, isNBC=isNBC, Pbar=P,
+                        counters=cnt, idx=idx, P=st["Pn"][:, idx], state=st["sn"][:, idx])
+    print("c3_128 P_bar_xx:", P[:, 0].tolist(), "counters:", cnt.tolist())
+
+
+C5_CYCLIC_32 = dict(N=32, ngrains=16, seed=20261202 + 55, nstep=720, strain=0.015, cycles=3,
+                    slip_set=80, run_steps=70)
+
+
+def c5_cyclic_workload(p):
+    """the Workload a c5_cyclic_32 parameter set describes"""
+    from cpfft_amd.configs import Material, Workload
+    return Workload("c5_cyclic_32", int(p["N"]), int(p["ngrains"]), int(p["seed"]),
+                    int(p["nstep"]), float(p["strain"]), cycles=int(p["cycles"]),
+                    material=Material(slip_set=int(p["slip_set"])))
+
+
+def c5small():
+    p = C5_CYCLIC_32
+    wl = c5_cyclic_workload(p)
+    BC, isNBC, ang, P, cnt, st = run(wl, nstep=p["run_steps"])
+    np.savez_compressed(os.path.join(HERE, "c5_cyclic_32_oracle.npz"), BC=BC, isNBC=isNBC,
+                        Pbar=P, counters=cnt, **{k: np.array(v) for k, v in p.items()})
+    print("c5_cyclic_32 P_bar_xx:", P[:, 0].tolist(), "counters:", cnt.tolist())
+
+
 def c4():
     members = BATCH_WORKLOADS["c4_batch"].members()
     keep = [k for k, m in enumerate(members) if m.N == 32]
@@ -85,4 +125,4 @@ def c4big():
 
 if __name__ == "__main__":
     for w in sys.argv[1:] or ["c2", "c4", "c4big"]:
-        {"c2": c2, "c4": c4, "c4big": c4big}[w]()
+        {"c2": c2, "c3": c3, "c4": c4, "c4big": c4big, "c5small": c5small}[w]()

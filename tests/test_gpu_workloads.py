@@ -1,8 +1,10 @@
 """GPU tests of the benchmark workloads at their full sizes (configs.py):
 
-* 64^3, 64 grains (c2_64): the first two load steps against the oracle's
-  run (tests/golden/c2_64_oracle.npz): P-bar within 1e-9, identical
-  Newton / CG / mixed-BC / update counts, sampled voxels within 1e-9.
+* 64^3, 64 grains (c2_64) and 128^3, 256 grains (c3_128, the benchmark's
+  workload): the first two load steps against the oracle's runs
+  (tests/golden/c2_64_oracle.npz, c3_128_oracle.npz): P-bar within 1e-9,
+  identical Newton / CG / mixed-BC / update counts, sampled voxels within
+  1e-9.
 * 128^3, 256 grains (c3_128) and 256^3, 1024 grains of BCC24 (c5_256): the
   first two load steps, checked by what needs no oracle run of the whole
   grid -- the mixed boundary condition met (P_yy = P_zz = 0 to the
@@ -10,6 +12,9 @@
   fresh solve formed again by a separate operator application -- and the
   stress update of the next step checked voxel by voxel against the
   oracle on a slab of 16384 voxels of the real state.
+* config 5's material and cyclic loading on a 32^3 RVE through its first
+  reversal, 70 load steps against the oracle's run
+  (tests/golden/c5_cyclic_32_oracle.npz).
 * the batch of 24 RVEs (c4_batch): two load steps of all 24 in one
   batched run, every 32^3 member against the oracle's run of it
   (tests/golden/c4_n32_oracle.npz); and the first load step of its first
@@ -85,24 +90,30 @@ def _check_cg_residual(s, wl):
     assert r <= max(wl.tolPCG * nb, wl.tolPCG) * 1.05
 
 
-def test_c2_64_first_steps_against_oracle_fixture():
-    g = np.load(os.path.join(GOLDEN, "c2_64_oracle.npz"))
-    wl = WORKLOADS["c2_64"]
+@pytest.mark.parametrize("name", ["c2_64", "c3_128"])
+def test_first_steps_against_oracle_fixture(name):
+    """The first two load steps of the 64^3 and the 128^3 (benchmark)
+    workloads against the oracle's run (tests/golden/make_workload_oracle.py):
+    P_bar, every count, and P and the material state of sampled voxels."""
+    g = np.load(os.path.join(GOLDEN, f"{name}_oracle.npz"))
+    wl = WORKLOADS[name]
     BC, isNBC = wl.bc()
     assert np.array_equal(g["BC"], BC[:2]) and np.array_equal(g["isNBC"], isNBC)
     s = C.Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
     rc, Pbar, cnt, _ = s.compute(BC[:2], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
-    print(f"\n[c2] P_xx {Pbar[:, 0].tolist()} counters {cnt.tolist()}", flush=True)
+    print(f"\n[{name}] P_xx {Pbar[:, 0].tolist()} counters {cnt.tolist()}", flush=True)
     assert rc == 0
     assert rel(Pbar, g["Pbar"]) < 1e-9
     assert np.array_equal(cnt, g["counters"])
     idx = g["idx"]
-    assert rel(s.download(C.PN)[:, idx], g["P"]) < 1e-9
+    eP = rel(s.download(C.PN)[:, idx], g["P"])
     sg, so = s.download(C.STATE_N)[:, idx], g["state"]
     blocks = {"stress": (0, 6), "Rp": (6, 15), "tt": (15, 17), "D": (17, 23),
               "slip": (23, 47), "work": (47, 50)}
     err = {k: rel(sg[a:b], so[a:b]) for k, (a, b) in blocks.items()}
-    print("[c2] state rel diffs " + ", ".join(f"{k} {v:.1e}" for k, v in err.items()), flush=True)
+    print(f"[{name}] sampled P rel diff {eP:.2e}; state rel diffs "
+          + ", ".join(f"{k} {v:.1e}" for k, v in err.items()), flush=True)
+    assert eP < 1e-9
     for k in ("stress", "Rp", "tt"):
         assert err[k] < 1e-9, k
     # Three fields amplify rounding, and after the 18 updates of the two
@@ -156,6 +167,32 @@ def test_c4_batch_first_steps_against_oracle_fixture():
         assert np.array_equal(cnt, g["counters"][j]), k
     for sv in solvers:
         sv.close()
+
+
+def test_c5_cyclic_bcc24_32_through_the_first_reversal():
+    """Config 5's material and loading (BCC24, cyclic +-1.5% in steps of
+    2.5e-4) on a 32^3 RVE of 16 grains for its first 70 load steps, past the
+    reversal after step 60, against the oracle's run
+    (tests/golden/c5_cyclic_32_oracle.npz): P_bar of every step within 1e-9
+    and identical Newton / CG / mixed-BC / update counts."""
+    g = np.load(os.path.join(GOLDEN, "c5_cyclic_32_oracle.npz"))
+    wl = C.Workload("c5_cyclic_32", int(g["N"]), int(g["ngrains"]), int(g["seed"]),
+                    int(g["nstep"]), float(g["strain"]), cycles=int(g["cycles"]),
+                    material=C.Material(slip_set=int(g["slip_set"])))
+    n = int(g["run_steps"])
+    BC, isNBC = wl.bc()
+    assert np.array_equal(BC[:n], g["BC"]) and np.array_equal(isNBC, g["isNBC"])
+    s = C.Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
+    rc, Pbar, cnt, _ = s.compute(BC[:n], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
+    print(f"\n[c5 cyclic 32] rc {rc} P_xx at steps 60, 61, 70: "
+          f"{Pbar[59, 0]:.4f} {Pbar[60, 0]:.4f} {Pbar[n - 1, 0]:.4f}, "
+          f"CG per step {cnt[:, 1].sum() / n:.0f}, P_bar rel diff {rel(Pbar, g['Pbar']):.1e}",
+          flush=True)
+    assert rc == 0
+    assert Pbar[59, 0] > Pbar[60, 0] > Pbar[n - 1, 0]  # unloading after the reversal
+    assert rel(Pbar, g["Pbar"]) < 1e-9
+    assert np.array_equal(cnt, g["counters"])
+    s.close()
 
 
 def _check_state_blocks(sg, so, ns, what):
