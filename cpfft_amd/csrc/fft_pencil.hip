@@ -93,11 +93,11 @@ constexpr bool regc_ok(int n) {
 }
 constexpr bool lds_fits(int lines, int n) { return (lines * lstride(n) + n) * 16 <= 65536; }
 // pass C: component lines per block, each candidate where its lines fit
-// the 64 KiB of static LDS.  Where regc_ok: kC_xr with the three components
-// of one tensor row for 8, 4 or 2 y lines (24, 12, 6).  Elsewhere: kC_x
-// with all nine components of 2 or 1 y lines (18, 9).  One kernel family
-// per N: kC_xr is the faster wherever it applies (at 128^3 60-66 us against
-// 85-88 us for kC_x).
+// the 64 KiB of static LDS.  Where regc_ok: first kC_xr with the three
+// components of one tensor row for 8, 4 or 2 y lines (24, 12, 6).  Then,
+// for every N, kC_x with all nine components of 2 or 1 y lines (18, 9).
+// Both compute the same bits; the autotuner picks (at 128^3 kC_xr takes
+// 60-66 us against 85-88 us for kC_x).
 constexpr int xl_cand(int n, int i) {
   int k = 0;
   if (regc_ok(n)) {
@@ -106,7 +106,6 @@ constexpr int xl_cand(int n, int i) {
         if (k == i) return 3 * yb;
         ++k;
       }
-    return 0;
   }
   for (int yb = 2; yb >= 1; --yb)
     if (lds_fits(9 * yb, n)) {
