@@ -71,16 +71,24 @@ def test_c3_128_fixture_inputs():
 
 
 def test_c5_cyclic_32_fixture_inputs():
-    """config 5's loading on 32^3: the parameters it stores give its inputs"""
+    """config 5's crystal and strain increment in a smaller cycle on 32^3:
+    the parameters it stores give its inputs"""
     g = _fixture("c5_cyclic_32_oracle.npz")
     from cpfft_amd.configs import Material, Workload
     wl5 = WORKLOADS["c5_256"]
     wl = Workload("c5_cyclic_32", int(g["N"]), int(g["ngrains"]), int(g["seed"]),
                   int(g["nstep"]), float(g["strain"]), cycles=int(g["cycles"]),
                   material=Material(slip_set=int(g["slip_set"])))
-    assert (wl.nstep, wl.strain, wl.cycles, wl.dt, wl.material) == (
-        wl5.nstep, wl5.strain, wl5.cycles, wl5.dt, wl5.material)
+    assert (wl.dt, wl.material) == (wl5.dt, wl5.material)
+
+    def increment(w):
+        return w.strain / (w.nstep // (4 * w.cycles))
+    assert abs(increment(wl) - increment(wl5)) < 1e-15
     n = int(g["run_steps"])
     BC, isNBC = wl.bc()
     assert np.array_equal(g["BC"], BC[:n]) and np.array_equal(g["isNBC"], isNBC)
     assert g["Pbar"].shape == (n, 9) and n > wl.nstep // (4 * wl.cycles)  # past the reversal
+    # the steps the oracle completed come first; it stops with rc < 0 or none
+    done = int(np.count_nonzero(g["counters"].any(axis=1)))
+    assert g["counters"][:done].all() and not g["counters"][done:].any()
+    assert (int(g["rc"]) < 0) == (done < n)

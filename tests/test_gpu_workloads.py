@@ -12,9 +12,9 @@
   fresh solve formed again by a separate operator application -- and the
   stress update of the next step checked voxel by voxel against the
   oracle on a slab of 16384 voxels of the real state.
-* config 5's material and cyclic loading on a 32^3 RVE through its first
-  reversal, 70 load steps against the oracle's run
-  (tests/golden/c5_cyclic_32_oracle.npz).
+* config 5's crystal and strain increment in a +-0.15% cycle on a 32^3
+  RVE up to its first reversal, where the reference algorithm stops,
+  against the oracle's run (tests/golden/c5_cyclic_32_oracle.npz).
 * the batch of 24 RVEs (c4_batch): two load steps of all 24 in one
   batched run, every 32^3 member against the oracle's run of it
   (tests/golden/c4_n32_oracle.npz); and the first load step of its first
@@ -177,12 +177,15 @@ def test_c4_batch_first_steps_against_oracle_fixture():
         sv.close()
 
 
-def test_c5_cyclic_bcc24_32_through_the_first_reversal():
-    """Config 5's material and loading (BCC24, cyclic +-1.5% in steps of
-    2.5e-4) on a 32^3 RVE of 16 grains for its first 70 load steps, past the
-    reversal after step 60, against the oracle's run
-    (tests/golden/c5_cyclic_32_oracle.npz): P_bar of every step within 1e-9
-    and identical Newton / CG / mixed-BC / update counts."""
+def test_c5_cyclic_bcc24_32_at_the_first_reversal():
+    """Config 5's crystal (BCC24) and strain increment (2.5e-4) in a
+    +-0.15% tension-compression cycle on a 32^3 RVE of 16 grains, load
+    steps 1 to 12, against the oracle's run
+    (tests/golden/c5_cyclic_32_oracle.npz).  The reference algorithm stops
+    at step 7, the first after the reversal (its Newton iteration diverges
+    and a CG solve exhausts its iterations); the GPU path stops there too,
+    with P_bar of steps 1-6 within 1e-9 and identical Newton / CG /
+    mixed-BC / update counts."""
     path = os.path.join(GOLDEN, "c5_cyclic_32_oracle.npz")
     if not os.path.exists(path):
         pytest.skip("c5_cyclic_32_oracle.npz not generated (tests/golden/make_workload_oracle.py)")
@@ -195,13 +198,17 @@ def test_c5_cyclic_bcc24_32_through_the_first_reversal():
     assert np.array_equal(BC[:n], g["BC"]) and np.array_equal(isNBC, g["isNBC"])
     s = C.Solver(wl.N, wl.angles(), wl.material, dt=wl.dt, gamma_mode=wl.gamma_mode)
     rc, Pbar, cnt, _ = s.compute(BC[:n], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
-    print(f"\n[c5 cyclic 32] rc {rc} P_xx at steps 60, 61, 70: "
-          f"{Pbar[59, 0]:.4f} {Pbar[60, 0]:.4f} {Pbar[n - 1, 0]:.4f}, "
-          f"CG per step {cnt[:, 1].sum() / n:.0f}, P_bar rel diff {rel(Pbar, g['Pbar']):.1e}",
-          flush=True)
-    assert rc == 0
-    assert Pbar[59, 0] > Pbar[60, 0] > Pbar[n - 1, 0]  # unloading after the reversal
-    assert rel(Pbar, g["Pbar"]) < 1e-9
+    q = wl.nstep // (4 * wl.cycles)  # steps to the first reversal
+    rco = int(g["rc"])
+    # steps completed: a step's counters are written when it ends
+    done = int(np.count_nonzero(g["counters"].any(axis=1)))
+    gdone = int(np.count_nonzero(cnt.any(axis=1)))
+    print(f"\n[c5 cyclic 32] reversal after step {q}; oracle rc {rco} after {done} steps, "
+          f"GPU rc {rc} after {gdone}; P_xx of steps 1-{done}: "
+          f"{np.round(Pbar[:done, 0], 3).tolist()}; P_bar rel diff "
+          f"{rel(Pbar[:done], g['Pbar'][:done]):.1e}", flush=True)
+    assert (rc < 0) == (rco < 0) and gdone == done
+    assert rel(Pbar[:done], g["Pbar"][:done]) < 1e-9
     assert np.array_equal(cnt, g["counters"])
     s.close()
 
