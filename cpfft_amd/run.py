@@ -6,6 +6,7 @@ and a summary line at the end.
 
     python -m cpfft_amd.run --workload c3_128                # all 240 steps
     python -m cpfft_amd.run --workload c5_256 --last 64 --group 1
+    python -m cpfft_amd.run --workload c3_128 --warm-previous
 
 A run stops at the first step the driver fails (rc < 0), as the reference
 stops at its die_abort paths.
@@ -16,7 +17,7 @@ import time
 
 import numpy as np
 
-from . import Solver
+from . import WARM_PREVIOUS, Solver
 from .configs import WORKLOADS
 
 
@@ -28,6 +29,9 @@ def main(argv=None):
     ap.add_argument("--group", type=int, default=10, help="load steps per cpfft_compute call")
     ap.add_argument("--no-graphs", action="store_true", help="launch the CG chunks eagerly")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--warm-previous", action="store_true",
+                    help="start the local solves from the step's previous global iterate "
+                         "(cpfft_set_warm_start with CPFFT_WARM_PREVIOUS)")
     args = ap.parse_args(argv)
     wl = WORKLOADS[args.workload]
     last = args.last or wl.nstep
@@ -40,6 +44,8 @@ def main(argv=None):
         s.set_graphs(True)
     if not args.no_autotune:
         s.autotune()
+    if args.warm_previous:
+        s.set_warm_start(WARM_PREVIOUS)
     print(f"[run] {wl.name}: N={wl.N}, {wl.ngrains} grains, steps {args.first}-{last}; "
           f"set up in {time.perf_counter() - t0:.1f} s, shapes {s.tune()}", flush=True)
     if args.first > 1:
@@ -47,6 +53,7 @@ def main(argv=None):
         rc, _, _, _ = s.compute(BC[:args.first - 1], isNBC, wl.tolPCG, wl.tolNR, wl.maxIter)
         if rc != 0:
             raise SystemExit(f"[run] the steps before {args.first} stopped with rc {rc}")
+    loc0 = s.local_stats()
     t1 = time.perf_counter()
     counts, rc, done = [], 0, args.first - 1
     for k0 in range(args.first - 1, last, args.group):
@@ -66,8 +73,11 @@ def main(argv=None):
     wall = time.perf_counter() - t1
     c = np.vstack(counts) if counts else np.zeros((0, 4), np.int32)
     n = c.shape[0]
+    loc = s.local_stats()
     print(json.dumps({
         "workload": wl.name, "steps": [args.first, done], "rc": int(rc), "seconds": wall,
+        "warm_start": "previous iterate" if args.warm_previous else None,
+        "local_iterations": int(loc[0] - loc0[0]),  # of the voxels, over the timed steps
         "load_steps_per_s": n / wall if wall > 0 else None,
         "per_step": {k: float(c[:, j].mean()) if n else None for j, k in
                      enumerate(("newton", "cg_iterations", "mixed_bc_loops", "drive_eps_sig"))},
