@@ -119,8 +119,8 @@ def test_first_steps_against_oracle_fixture(name):
     # Per-voxel P after two load steps (4000+ CG iterations) agrees to about
     # 1e-9 between any two correct FP64 runs that differ only in rounding
     # order: an oracle whose CG sums run pairwise instead of in one running
-    # sum differs from this fixture's oracle by 8.7e-10 at 64^3.  The GPU is
-    # at 9.6e-10 there and 1.29e-9 at 128^3 (larger grid, more iterations).
+    # sum differs from this fixture's oracle by 8.7e-10 at 64^3 and 1.26e-9
+    # at 128^3.  The GPU is at 9.6e-10 and 1.29e-9.
     assert eP < {"c2_64": 1e-9, "c3_128": 2e-9}[name]
     for k in ("stress", "Rp", "tt"):
         assert err[k] < 1e-9, k
