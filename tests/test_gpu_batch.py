@@ -141,8 +141,8 @@ def test_batch_continues_runs():
 @pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128, 256])
 def test_every_tuned_shape_gives_identical_results(N):
     """The autotuner's candidates (cpfft_set_tune accepts exactly those)
-    change only which block transforms which lines, and for N = 64 and 128
-    whether pass C runs its transforms in LDS or in registers: G_K_dF with
+    change only which block transforms which lines, and for N = 64, 128 and
+    256 whether pass C runs its transforms in LDS or in registers: G_K_dF with
     and without K4 and a fused CG solve are bitwise equal under every one of
     them."""
     s = _solver(N)
